@@ -1,0 +1,162 @@
+/*
+ * oracle.h -- CPU restatement of Cardioid's electrics hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (cardioid_amd/, the
+ * HIP library, bench.py's GPU leg) may link, load or call this code.  It is
+ * the checker for the HIP path (tests/, __graft_entry__.smoke()) and the
+ * CPU baseline leg of bench.py.
+ *
+ * What it restates (file:line in the reference tree):
+ *   - TT06Dev non-gate update          elec/TT06NonGates.c:374-563
+ *   - TT06 constants                   elec/TT06NonGates.c:11-251
+ *   - exact voltage functions fv0..fv6  elec/TT06NonGates.c:319-364
+ *   - rational-fit evaluation          elec/pade.cc:17-36
+ *   - gate updates, "v2" semantics     elec/TT06GatesSimd.c (per gate, see tt06_oracle.c)
+ *   - gate updates, generic/exact      elec/TT06Func.cc:64-161, 261-417
+ *   - dt scaling of the tauR fits      elec/TT06Dev_Reaction.cc:130-136
+ *   - cell types and rest states       elec/TT06Func.cc:421-612
+ *   - FGR weights                      elec/FGRDiffusionOMP.cc:166-265, elec/FGRUtils.cc:10-148
+ *   - FGR stencil                      elec/FGRDiffusionOMP.cc:109-131
+ *   - omp-loop step                    elec/simulationLoop.cc:170-241
+ *   - box stimulus / periodic pulse    elec/BoxStimulus.cc:18-64, elec/PeriodicPulse.cc:22-28,
+ *                                      elec/Stimulus.hh:21-27
+ *   - activation sensors               elec/ActivationTimeSensor.cc:99-111,
+ *                                      elec/ActivationAndRecoverySensor.cc:76-92
+ *
+ * How each part is pinned is recorded in tests/ (test_oracle_*.py).
+ */
+#ifndef CARDIOID_ORACLE_H
+#define CARDIOID_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* State slots, same order as TT06_DEV_STATE_INDEX (elec/TT06Func.h:10-14). */
+enum {
+   ORA_CA_I = 0, ORA_K_I, ORA_NA_I, ORA_CA_SS, ORA_CA_SR, ORA_R_PRIME, ORA_FCASS,
+   ORA_M, ORA_H, ORA_J, ORA_XR1, ORA_XR2, ORA_XS, ORA_R, ORA_D, ORA_F, ORA_F2,
+   ORA_JL, ORA_S, ORA_NSTATE
+};
+/* ORA_K_I holds dVK_i = K_i/c9 + Vm, as the reference does (TT06Func.h:9). */
+
+#define ORA_NFIT 33
+#define ORA_MAXCOEF 40
+
+/* One rational fit: coef[0..m-1] numerator (ascending powers), coef[m] = 1,
+ * coef[m+1..m+l-1] denominator.  l < 2 means polynomial only. */
+typedef struct {
+   int l, m;
+   double coef[ORA_MAXCOEF];
+} ora_pade;
+
+/* Fit set in the reference's order: fv0..fv6, then (mhu, tauR) for the
+ * gates m, h, j, Xr1, Xr2, Xs, r, d, f, f2, jL, s0, s1. */
+typedef struct {
+   ora_pade fit[ORA_NFIT];
+   int dt_scaled;           /* 1 once the tauR numerators were multiplied by dt */
+} ora_fitset;
+
+typedef struct {
+   int s_switch;
+   double P_NaK, g_Ks, g_Kr, g_to, g_NaL;
+   double Vm0;
+   double state0[ORA_NSTATE];   /* K_i (not dVK_i) in slot ORA_K_I */
+} ora_celltype;
+
+/* Per-cell drug-block scales on g_Kr, g_CaL, g_Na (1.0 = reference). */
+typedef struct { double kr, cal, na; } ora_scales;
+
+/* ---- constants / cell types ---- */
+void   ora_tt06_init(void);                 /* initNonGateCnst + exp table */
+double ora_c9(void);
+int    ora_celltype_standard(const char *name, ora_celltype *out);
+
+/* ---- fits ---- */
+double ora_pade_eval(const ora_pade *p, double x);
+void   ora_fitset_scale_dt(ora_fitset *fs, double dt);
+
+/* ---- reaction, configuration B (rational fits, v2 gates) ----
+ * state is SoA: state[s*stride + i].  Vm and dVdt are indexed by i. */
+void ora_tt06_nongate(const ora_fitset *fs, double dt, const ora_celltype *ct,
+                      const ora_scales *sc, int n, const double *Vm,
+                      double *state, int64_t stride, double *dVdt);
+void ora_tt06_gates_v2(const ora_fitset *fs, int s_switch, int n,
+                       const double *Vm, double *state, int64_t stride);
+/* flags of cell i of a reaction group of n cells: bit0 gates fixed (v2 loops
+ * stop at 4*floor(n/4)), bit1 f gate in the main-loop lane-0..3 form */
+int  ora_gate_flags(int64_t i, int64_t n);
+/* one gate only (eq 0..12 in fit order m,h,j,Xr1,Xr2,Xs,r,d,f,f2,jL,s0,s1) */
+void ora_tt06_gate_v2(const ora_fitset *fs, int eq, int n, const double *Vm, double *g);
+
+/* ---- reaction, configuration A (tolerance 0: exact functions) ---- */
+void ora_tt06_nongate_exact(double dt, const ora_celltype *ct, int n,
+                            const double *Vm, double *state, int64_t stride,
+                            double *dVdt);
+void ora_tt06_gates_exact(double dt, int s_switch, int jhTauSmooth, int n,
+                          const double *Vm, double *state, int64_t stride);
+double ora_tt06_exact_func(int idx, double V);   /* idx in fit order, 0..32 */
+
+/* init state for n cells (SoA with stride), dVK_i computed as the reference does */
+void ora_tt06_init_state(const ora_celltype *ct, int n, double *state,
+                         int64_t stride, double *Vm);
+
+/* ---- FGR diffusion ----
+ * Box: dims (bx,by,bz), z fastest: idx = z + bz*(y + by*x) (Array3d.hh:173-177).
+ * tissue[] over the box (0/1); sigma[6*idx+k] = a11,a12,a13,a22,a23,a33. */
+void ora_fgr_offsets(int bx, int by, int bz, int64_t off[19]);
+void ora_fgr_weights(int bx, int by, int bz, const uint8_t *tissue,
+                     const double *sigma, double dx, double dy, double dz,
+                     int64_t ncell, const int64_t *blockIndex,
+                     float *A /* 19*ncell, cell-major */, double *A0 /* ncell */);
+void ora_fgr_f2(unsigned iFace, const int tissue[19], double gradPhi[3][19]);
+void ora_fgr_calc(int64_t ncell, const int64_t *blockIndex, const float *A,
+                  const double *A0, const int64_t off[19], const double *VmBlock,
+                  double scale, double *dVm);
+
+/* ---- whole omp-loop simulation ---- */
+typedef struct ora_sim ora_sim;
+
+typedef struct {
+   int nx, ny, nz;              /* global grid; gid = x + nx*(y + ny*z) */
+   double dx, dy, dz, dt, diffusionScale;
+   int64_t ncell;               /* tissue cells */
+   const int64_t *gid;          /* ncell, ascending */
+   const uint8_t *celltype;     /* ncell, index into celltypes[] */
+   const double *sigma;         /* 6*ncell */
+   int ncelltype;
+   const ora_celltype *celltypes;
+   const ora_scales *scales;    /* ncell or NULL */
+   const ora_fitset *fits;      /* unscaled fit set (scaled internally) */
+   int exact;                   /* 1: configuration A */
+   double threshold;            /* ActivationAndRecovery threshold (mV) */
+   int nthreads;
+} ora_sim_params;
+
+typedef struct {
+   int xMin, xMax, yMin, yMax, zMin, zMax;   /* strict box in grid tuples */
+   double t0, tf;                             /* Stimulus window (open) */
+   double period, duration, amplitude, phase; /* PeriodicPulse; amplitude = -vStim */
+} ora_box_stim;
+
+ora_sim *ora_sim_create(const ora_sim_params *p, int nstim, const ora_box_stim *stim);
+void ora_sim_destroy(ora_sim *s);
+void ora_sim_run(ora_sim *s, int nsteps);
+void ora_sim_get_vm(const ora_sim *s, double *out);
+void ora_sim_get_state(const ora_sim *s, double *out /* 19*ncell SoA */);
+void ora_sim_set_state(ora_sim *s, const double *in, const double *Vm);
+/* first time Vm > 0 (ActivationTimeSensor), -1 if never */
+void ora_sim_get_activation_vm0(const ora_sim *s, double *out);
+/* first up-crossing of the threshold (ActivationAndRecovery), -1 if never */
+void ora_sim_get_activation_thr(const ora_sim *s, double *out);
+/* first down-crossing after first activation, -1 if never */
+void ora_sim_get_recovery_thr(const ora_sim *s, double *out);
+double ora_sim_time(const ora_sim *s);
+int ora_sim_loop(const ora_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
