@@ -1,0 +1,155 @@
+// kernels.hpp -- kernel argument blocks and small device helpers shared by
+// the FP64 and mixed-precision kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tt06_model.hpp"
+
+namespace cdh {
+
+struct ParamSet;
+
+constexpr int kMaxStim = 8;
+
+// Reaction over n cells of one reaction group (drop-in object).
+struct ReactionArgs {
+   TT06Const K;
+   const double *coef;          // 468 dt-scaled fit coefficients
+   const ParamSet *params;      // parameter sets
+   const uint8_t *pid;          // per-cell parameter set, or null (set 0)
+   const uint8_t *flags;        // per-cell gate flags
+   double *state;               // [19][stride]
+   int64_t stride;
+   int n;
+   const int *index;            // reaction-local -> array index, or null (identity)
+   const double *Vm;
+   double *dVm;
+   double dt;
+};
+
+struct StencilArgs {
+   int n;
+   int64_t stride;
+   const uint32_t *bi;          // box index per cell
+   const double *box;           // Vm in the padded box
+   const double *A0;            // centre weight (double)
+   const float *Aw;             // [18][stride] off-centre weights, NodeLocation order 1..18
+   int off[19];                 // NodeLocation offsets in the box
+   double scale;                // diffusionScale
+   double *dVm;
+};
+
+// Stimulus values of one step: value[k] applies where bit k of a cell's
+// mask is set and bit k of `on` is set; later stimuli overwrite earlier
+// ones, as BoxStimulus assigns (elec/BoxStimulus.cc:61-62).
+struct StimStep {
+   int nstim;
+   uint32_t on;
+   double value[kMaxStim];
+};
+
+__device__ __forceinline__ double stim_value(const StimStep &s, uint8_t mask)
+{
+   double v = 0.0;
+#pragma unroll
+   for (int k = 0; k < kMaxStim; k++)
+      if (k < s.nstim && ((mask >> k) & 1u) && ((s.on >> k) & 1u)) v = s.value[k];
+   return v;
+}
+
+// Activation sensors, evaluated after the step at the new time.
+//   tact0: first time Vm > 0            (ActivationTimeSensor.cc:99-111)
+//   tact / trec: first up-crossing of the threshold, and the first
+//   down-crossing after it                (ActivationAndRecoverySensor.cc:76-92)
+struct SensorArgs {
+   int n;
+   int enabled;
+   int first_eval;              // 1 on the first evaluation after the start
+   double time;
+   double threshold;
+   double *tact0;
+   double *tact;
+   double *trec;
+   uint8_t *active;             // threshold state (stateful form only)
+   uint8_t *activated;          // Vm > 0 seen (stateful form only)
+};
+
+// Stateful form, the reference's own bookkeeping.
+__device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v)
+{
+   if (!a.activated[i] && v > 0) {
+      a.activated[i] = 1;
+      a.tact0[i] = a.time;
+   }
+   if (a.active[i] && v < a.threshold) {
+      a.active[i] = 0;
+      if (a.trec[i] < 0 && a.tact[i] >= 0) a.trec[i] = a.time;
+   }
+   if (!a.active[i] && v > a.threshold) {
+      a.active[i] = 1;
+      if (a.tact[i] < 0) a.tact[i] = a.time;
+   }
+}
+
+// Pair form from (Vm before, Vm after) of the step: the state before the
+// step is "Vm above the level" (false before the first evaluation), which
+// equals the stateful form unless Vm lands exactly on the level.
+__device__ __forceinline__ void sensor_eval_pair(const SensorArgs &a, int i, double v0, double v1)
+{
+   const bool was0 = !a.first_eval && v0 > 0;
+   if (!was0 && v1 > 0 && a.tact0[i] < 0) a.tact0[i] = a.time;
+   const bool wasT = !a.first_eval && v0 > a.threshold;
+   if (!wasT && v1 > a.threshold && a.tact[i] < 0) a.tact[i] = a.time;
+   if (wasT && v1 < a.threshold && a.trec[i] < 0 && a.tact[i] >= 0) a.trec[i] = a.time;
+}
+
+struct FusedArgs {
+   TT06Const K;
+   const double *coef;
+   const ParamSet *params;
+   const uint8_t *pid;
+   const uint8_t *flags;
+   double *state;
+   int64_t stride;
+   int n;
+   int nblock_per_xcd;
+   const uint32_t *bi;
+   const double *box_cur;
+   double *box_next;
+   const double *A0;
+   const float *Aw;
+   int off[19];
+   double scale;
+   double dt;
+   int stim_any;
+   const uint8_t *stim_mask;
+   StimStep stim;
+   SensorArgs sens;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD owns one contiguous range of cells (and of the box), which keeps the
+// stencil's neighbour rows in that XCD's L2.
+__device__ __forceinline__ int xcd_cell_index(int n, int nblock_per_xcd)
+{
+   const int wg = blockIdx.x;
+   const int logical = (wg & 7) * nblock_per_xcd + (wg >> 3);
+   return logical * blockDim.x + threadIdx.x;
+}
+
+// kernels (kernels_f64.hip)
+__global__ void k_tt06_calc_f64(ReactionArgs a);
+__global__ void k_tt06_gate_f64(ReactionArgs a);
+__global__ void k_scatter_f64(int n, const uint32_t *bi, const double *src, double *box);
+__global__ void k_fgr_calc_f64(StencilArgs a);
+__global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR,
+                                const double *dVmD, const double *iStim);
+__global__ void k_fill_f64(int n, double v, double *x);
+__global__ void k_set_list_f64(int n, const int *list, double v, double *x);
+__global__ void k_sensor_f64(SensorArgs a, const double *Vm);
+__global__ void k_step_fused_f64(FusedArgs a);
+
+}  // namespace cdh

@@ -1,0 +1,153 @@
+// kernels_f64.hip -- FP64 kernels of the drop-in path (gfx950).
+//
+// Compiled with -ffp-contract=off: every kernel here reproduces the
+// reference CPU path's rounding, operation by operation.
+//
+//   k_tt06_calc_f64     TT06Dev_Reaction::calc          elec/TT06Dev_Reaction.cc:393-407
+//   k_scatter_f64       FGRDiffusionOMP::updateLocalVoltage  elec/FGRDiffusionOMP.cc:75-89
+//   k_fgr_calc_f64      FGRDiffusionOMP::calc           elec/FGRDiffusionOMP.cc:109-131
+//   k_integrate_f64     Vm += dt*(dVmR+dVmD+iStim)      elec/simulationLoop.cc:221-227
+//   k_fill_f64 / k_set_list_f64   iStim = 0; BoxStimulus  elec/simulationLoop.cc:186-193,
+//                                                   elec/BoxStimulus.cc:53-64
+//   k_step_fused_f64    all of the above for one omp-loop step, one launch
+//   k_sensor_f64        ActivationTimeSensor / ActivationAndRecoverySensor eval
+#pragma clang fp contract(off)
+
+#include "kernels.hpp"
+#include "tt06_f64.hpp"
+
+namespace cdh {
+
+__global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.n) return;
+   const int e = a.index ? a.index[i] : i;
+   const double V = a.Vm[e];
+   double st[N_STATE];
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
+   const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
+   const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
+   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st);
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
+   a.dVm[e] = dVR;
+}
+
+// Gate update only (Reaction::updateGate of the pdr loop): the 12 gates at Vm.
+__global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.n) return;
+   const int e = a.index ? a.index[i] : i;
+   const double V = a.Vm[e];
+   double st[N_STATE];
+#pragma unroll
+   for (int s = S_M; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
+   const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
+   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st);
+#pragma unroll
+   for (int s = S_M; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
+}
+
+__global__ __launch_bounds__(256) void k_scatter_f64(int n, const uint32_t *__restrict__ bi,
+                                                     const double *__restrict__ src,
+                                                     double *__restrict__ box)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) box[bi[i]] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.n) return;
+   const double *phi = a.box + a.bi[i];
+   double tmp = a.A0[i] * phi[0];
+#pragma unroll
+   for (int k = 1; k < 19; k++) tmp += (double)a.Aw[(k - 1) * a.stride + i] * phi[a.off[k]];
+   a.dVm[i] = tmp * a.scale;
+}
+
+__global__ __launch_bounds__(256) void k_integrate_f64(int n, double dt, double *__restrict__ Vm,
+                                                       const double *__restrict__ dVmR,
+                                                       const double *__restrict__ dVmD,
+                                                       const double *__restrict__ iStim)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) Vm[i] += dt * (dVmR[i] + dVmD[i] + iStim[i]);
+}
+
+__global__ __launch_bounds__(256) void k_fill_f64(int n, double v, double *__restrict__ x)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) x[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_set_list_f64(int n, const int *__restrict__ list, double v,
+                                                      double *__restrict__ x)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) x[list[i]] = v;
+}
+
+// Integrator fused with the activation sensors (pair form: Vm before and
+// after the step are both at hand here).
+__global__ __launch_bounds__(256) void k_integrate_sensor_f64(int n, double dt,
+                                                              double *__restrict__ Vm,
+                                                              const double *__restrict__ dVmR,
+                                                              const double *__restrict__ dVmD,
+                                                              const double *__restrict__ iStim,
+                                                              SensorArgs s)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   const double v0 = Vm[i];
+   const double v1 = v0 + dt * (dVmR[i] + dVmD[i] + iStim[i]);
+   Vm[i] = v1;
+   if (s.enabled) sensor_eval_pair(s, i, v0, v1);
+}
+
+// Sensor evaluation at (time, loop) after the step; Vm in cell order.
+__global__ __launch_bounds__(256) void k_sensor_f64(SensorArgs a, const double *__restrict__ Vm)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.n) return;
+   sensor_eval(a, i, Vm[i]);
+}
+
+// One omp-loop step for every tissue cell, Vm in the padded box layout and
+// double buffered: reads box_cur, writes box_next.  Per cell, in order:
+// stimulus value, reaction at Vm_t, 19-point stencil on Vm_t, integrator.
+__global__ __launch_bounds__(256) void k_step_fused_f64(FusedArgs a)
+{
+   const int i = xcd_cell_index(a.n, a.nblock_per_xcd);
+   if (i >= a.n) return;
+   const uint32_t b = a.bi[i];
+   const double *phi = a.box_cur + b;
+   const double V = phi[0];
+
+   double st[N_STATE];
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
+   const ParamSet P = a.params[a.pid[i]];
+   const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
+   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st);
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
+
+   double tmp = a.A0[i] * phi[0];
+#pragma unroll
+   for (int k = 1; k < 19; k++) tmp += (double)a.Aw[(k - 1) * a.stride + i] * phi[a.off[k]];
+   const double dVD = tmp * a.scale;
+
+   double iStim = 0.0;
+   if (a.stim_any) iStim = stim_value(a.stim, a.stim_mask[i]);
+
+   const double Vn = V + a.dt * (dVR + dVD + iStim);
+   a.box_next[b] = Vn;
+   if (a.sens.enabled) sensor_eval_pair(a.sens, i, V, Vn);
+}
+
+}  // namespace cdh

@@ -1,0 +1,464 @@
+// plugins.hip -- the drop-in objects behind the C ABI: the TT06Dev reaction,
+// the FGR diffusion, the loop helpers and the activation sensor.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cardioid_hip.h"
+#include "fgr_weights.hpp"
+#include "hip_util.hpp"
+#include "kernels.hpp"
+#include "setup.hpp"
+#include "tt06_f64.hpp"
+
+namespace cdh {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Reaction
+// ---------------------------------------------------------------------------
+struct TT06Reaction {
+   int n = 0;
+   int64_t stride = 0;
+   TT06Const K;
+   CellType ct;
+   DevBuf<double> coef;
+   DevBuf<ParamSet> params;
+   DevBuf<uint8_t> flags;
+   DevBuf<double> state;
+   DevBuf<uint8_t> flagsAllFixed;   // gates skipped (updateNonGate-only launches)
+};
+
+}  // namespace cdh
+
+using namespace cdh;
+
+struct cdh_tt06 : TT06Reaction {};
+struct cdh_fgr {
+   int64_t nLocal = 0, nRemote = 0;
+   int64_t stride = 0;
+   BoxGeom geom;
+   StencilArgs args;
+   DevBuf<uint32_t> bi;       // local then remote
+   DevBuf<double> box, A0;
+   DevBuf<float> Aw;
+   double scale = 0.0;
+};
+struct cdh_sensor {
+   int64_t n = 0;
+   double threshold = -40.0;
+   int evaluated = 0;
+   DevBuf<double> tact0, tact, trec;
+   DevBuf<uint8_t> active, activated;
+};
+
+namespace {
+
+hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+ReactionArgs reaction_args(cdh_tt06 *r, double dt, const int *index, int n, const double *Vm,
+                           double *dVm, const uint8_t *flags)
+{
+   ReactionArgs a;
+   a.K = r->K;
+   a.coef = r->coef.p;
+   a.params = r->params.p;
+   a.pid = nullptr;
+   a.flags = flags;
+   a.state = r->state.p;
+   a.stride = r->stride;
+   a.n = n;
+   a.index = index;
+   a.Vm = Vm;
+   a.dVm = dVm;
+   a.dt = dt;
+   return a;
+}
+
+const char *kFieldName[N_STATE] = {"Ca_i", "dVK_i", "Na_i", "Ca_ss", "Ca_SR", "R_prime",
+                                   "fCass_gate", "m_gate", "h_gate", "j_gate", "Xr1_gate",
+                                   "Xr2_gate", "Xs_gate", "r_gate", "d_gate", "f_gate",
+                                   "f2_gate", "jL_gate", "s_gate"};
+// units as TT06Dev_Reaction::getHandleMap lists them (:533-551)
+const char *kFieldUnit[N_STATE] = {"mM", "mV", "mM", "mM", "mM", "1", "1", "1", "1", "1",
+                                   "1", "1", "1", "1", "1", "1", "1", "mM", "1"};
+
+}  // namespace
+
+extern "C" {
+
+const char *cdh_last_error(void) { return g_last_error.c_str(); }
+int cdh_version(void) { return 1; }
+
+int cdh_celltype_standard(const char *name, cdh_celltype *out)
+{
+   return guard([&] {
+      CDH_REQUIRE(name && out, "cdh_celltype_standard: null argument");
+      CellType ct;
+      if (!tt06_standard_celltype(name, &ct))
+         throw Error(kErrArg, std::string("unknown cell type ") + name);
+      static_assert(sizeof(cdh_celltype) == sizeof(CellType), "cell type layout");
+      std::memcpy(out, &ct, sizeof(ct));
+   });
+}
+
+int cdh_tt06_create(const cdh_tt06_params *p, int numPoints, cdh_tt06 **out)
+{
+   return guard([&] {
+      CDH_REQUIRE(p && out && numPoints >= 0, "cdh_tt06_create: bad arguments");
+      check_configuration_b(p->tolerance, p->mod, p->fastGate, p->fastNonGate);
+      auto r = std::make_unique<cdh_tt06>();
+      r->n = numPoints;
+      r->stride = (numPoints + 63) / 64 * 64;
+      tt06_constants(&r->K);
+      std::memcpy(&r->ct, &p->celltype, sizeof(CellType));
+      std::vector<double> coef = scaled_fit_coefficients(p->fit_l, p->fit_m, p->fit_coef, p->dt);
+      r->coef.alloc(coef.size());
+      r->coef.upload(coef.data(), coef.size());
+      ParamSet ps = make_param_set(r->ct, p->s_kr, p->s_cal, p->s_na);
+      r->params.alloc(1);
+      r->params.upload(&ps, 1);
+      std::vector<uint8_t> fl(r->stride, kFlagGatesFixed), fixed(r->stride, kFlagGatesFixed);
+      for (int i = 0; i < numPoints; i++) fl[i] = gate_flags(i, numPoints);
+      r->flags.alloc(r->stride);
+      r->flags.upload(fl.data(), fl.size());
+      r->flagsAllFixed.alloc(r->stride);
+      r->flagsAllFixed.upload(fixed.data(), fixed.size());
+      std::vector<double> st = rest_state_soa(r->ct, numPoints, r->stride, r->K.c9);
+      r->state.alloc(st.size());
+      r->state.upload(st.data(), st.size());
+      *out = r.release();
+   });
+}
+
+int cdh_tt06_destroy(cdh_tt06 *r)
+{
+   delete r;
+   return CDH_OK;
+}
+
+int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const double *Vm,
+                  const double *iStim, double *dVm, void *stream)
+{
+   (void)iStim;
+   return guard([&] {
+      CDH_REQUIRE(r && Vm && dVm, "cdh_tt06_calc: null argument");
+      CDH_REQUIRE(n == r->n, "cdh_tt06_calc: n differs from numPoints");
+      if (n == 0) return;
+      ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVm, r->flags.p);
+      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_tt06_update_nongate(cdh_tt06 *r, double dt, const int *indexArray, int n,
+                            const double *Vm, double *dVR, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(r && Vm && dVR && n == r->n, "cdh_tt06_update_nongate: bad arguments");
+      if (n == 0) return;
+      // the calc kernel with every gate held fixed is the non-gate update
+      ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVR, r->flagsAllFixed.p);
+      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_tt06_update_gate(cdh_tt06 *r, double dt, const int *indexArray, int n, const double *Vm,
+                         void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(r && Vm && n == r->n, "cdh_tt06_update_gate: bad arguments");
+      if (n == 0) return;
+      ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, nullptr, r->flags.p);
+      hipLaunchKernelGGL(k_tt06_gate_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_tt06_initialize_membrane_voltage(cdh_tt06 *r, const int *indexArray, int n, double *Vm,
+                                         void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(r && Vm && n == r->n, "cdh_tt06_initialize_membrane_voltage: bad arguments");
+      if (n == 0) return;
+      if (indexArray) {
+         hipLaunchKernelGGL(k_set_list_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream),
+                            n, indexArray, r->ct.Vm0, Vm);
+      } else {
+         hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), n,
+                            r->ct.Vm0, Vm);
+      }
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_tt06_checkpoint_field(int i, const char **name, const char **unit, int *handle)
+{
+   if (i < 0 || i >= N_STATE) {
+      set_last_error("cdh_tt06_checkpoint_field: index out of range");
+      return CDH_ERR_ARG;
+   }
+   if (name) *name = kFieldName[i];
+   if (unit) *unit = kFieldUnit[i];
+   if (handle) *handle = i;
+   return CDH_OK;
+}
+
+int cdh_tt06_var_handle(const char *name)
+{
+   if (!name) return -1;
+   for (int i = 0; i < N_STATE; i++)
+      if (std::strcmp(kFieldName[i], name) == 0) return i;
+   return -1;
+}
+
+int cdh_tt06_get_value(cdh_tt06 *r, int iCell, int handle, double *value)
+{
+   return guard([&] {
+      CDH_REQUIRE(r && value && iCell >= 0 && iCell < r->n && handle >= 0 && handle < N_STATE,
+                  "cdh_tt06_get_value: bad arguments");
+      r->state.download(value, 1, handle * r->stride + iCell);
+   });
+}
+
+int cdh_tt06_set_value(cdh_tt06 *r, int iCell, int handle, double value)
+{
+   return guard([&] {
+      CDH_REQUIRE(r && iCell >= 0 && iCell < r->n && handle >= 0 && handle < N_STATE,
+                  "cdh_tt06_set_value: bad arguments");
+      r->state.upload(&value, 1, handle * r->stride + iCell);
+   });
+}
+
+int cdh_tt06_get_state(cdh_tt06 *r, double *host)
+{
+   return guard([&] {
+      CDH_REQUIRE(r && host, "cdh_tt06_get_state: null argument");
+      CDH_HIP(hipDeviceSynchronize());
+      for (int s = 0; s < N_STATE; s++) r->state.download(host + (size_t)s * r->n, r->n, s * r->stride);
+   });
+}
+
+int cdh_tt06_set_state(cdh_tt06 *r, const double *host)
+{
+   return guard([&] {
+      CDH_REQUIRE(r && host, "cdh_tt06_set_state: null argument");
+      CDH_HIP(hipDeviceSynchronize());
+      for (int s = 0; s < N_STATE; s++) r->state.upload(host + (size_t)s * r->n, r->n, s * r->stride);
+   });
+}
+
+// ---------------------------------------------------------------------------
+// Diffusion
+// ---------------------------------------------------------------------------
+int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
+{
+   return guard([&] {
+      CDH_REQUIRE(p && out && p->gid && p->sigma && p->cellType, "cdh_fgr_create: null argument");
+      CDH_REQUIRE(p->nLocal >= 0 && p->nRemote >= 0, "cdh_fgr_create: negative sizes");
+      auto d = std::make_unique<cdh_fgr>();
+      const int64_t ntot = p->nLocal + p->nRemote;
+      d->nLocal = p->nLocal;
+      d->nRemote = p->nRemote;
+      d->stride = (p->nLocal + 63) / 64 * 64;
+      d->scale = p->diffusionScale;
+      d->geom = bounding_box(p->nx, p->ny, p->nz, p->gid, p->nLocal, p->dx, p->dy, p->dz);
+      const BoxGeom &g = d->geom;
+      const size_t nbox = (size_t)g.bx * g.by * g.bz;
+      CDH_REQUIRE(nbox < (size_t)UINT32_MAX, "cdh_fgr_create: box too large for 32-bit indices");
+      std::vector<int64_t> bidx(ntot);
+      std::vector<uint8_t> tissue(nbox, 0);
+      std::vector<double> sig(6 * nbox, 0.0);
+      for (int64_t c = 0; c < ntot; c++) {
+         int64_t ib = box_index(g, p->nx, p->ny, p->gid[c]);
+         CDH_REQUIRE(ib >= 0, "cdh_fgr_create: a remote cell lies outside the padded box");
+         bidx[c] = ib;
+         tissue[ib] = p->cellType[c] > 0 ? 1 : 0;
+         std::memcpy(&sig[6 * ib], p->sigma + 6 * c, 6 * sizeof(double));
+      }
+      std::vector<double> A0(d->stride, 0.0);
+      std::vector<float> Aw(18 * d->stride, 0.0f);
+      fgr_weights(g, tissue.data(), sig.data(), p->nLocal, bidx.data(), A0.data(), Aw.data(),
+                  d->stride);
+      std::vector<uint32_t> b32(ntot);
+      for (int64_t c = 0; c < ntot; c++) b32[c] = (uint32_t)bidx[c];
+      d->bi.alloc(ntot ? ntot : 1);
+      d->bi.upload(b32.data(), ntot);
+      d->box.alloc(nbox);
+      d->box.zero();
+      d->A0.alloc(d->stride ? d->stride : 1);
+      d->A0.upload(A0.data(), d->stride);
+      d->Aw.alloc(18 * (d->stride ? d->stride : 1));
+      d->Aw.upload(Aw.data(), Aw.size());
+      int64_t off[19];
+      fgr_offsets(g.bx, g.by, g.bz, off);
+      StencilArgs &a = d->args;
+      a.n = (int)p->nLocal;
+      a.stride = d->stride;
+      a.bi = d->bi.p;
+      a.box = d->box.p;
+      a.A0 = d->A0.p;
+      a.Aw = d->Aw.p;
+      for (int k = 0; k < 19; k++) a.off[k] = (int)off[k];
+      a.scale = d->scale;
+      a.dVm = nullptr;
+      *out = d.release();
+   });
+}
+
+int cdh_fgr_destroy(cdh_fgr *d)
+{
+   delete d;
+   return CDH_OK;
+}
+
+int cdh_fgr_update_local_voltage(cdh_fgr *d, const double *VmLocal, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(d && VmLocal, "cdh_fgr_update_local_voltage: null argument");
+      if (d->nLocal == 0) return;
+      hipLaunchKernelGGL(k_scatter_f64, dim3(grid_for(d->nLocal, kBlock)), dim3(kBlock), 0,
+                         S(stream), (int)d->nLocal, d->bi.p, VmLocal, d->box.p);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_fgr_update_remote_voltage(cdh_fgr *d, const double *VmRemote, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(d, "cdh_fgr_update_remote_voltage: null argument");
+      if (d->nRemote == 0) return;
+      CDH_REQUIRE(VmRemote, "cdh_fgr_update_remote_voltage: null argument");
+      hipLaunchKernelGGL(k_scatter_f64, dim3(grid_for(d->nRemote, kBlock)), dim3(kBlock), 0,
+                         S(stream), (int)d->nRemote, d->bi.p + d->nLocal, VmRemote, d->box.p);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_fgr_calc(cdh_fgr *d, double *dVm, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(d && dVm, "cdh_fgr_calc: null argument");
+      if (d->nLocal == 0) return;
+      StencilArgs a = d->args;
+      a.dVm = dVm;
+      hipLaunchKernelGGL(k_fgr_calc_f64, dim3(grid_for(d->nLocal, kBlock)), dim3(kBlock), 0,
+                         S(stream), a);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+double cdh_fgr_diffusion_scale(const cdh_fgr *d) { return d ? d->scale : 0.0; }
+
+// ---------------------------------------------------------------------------
+// Loop pieces
+// ---------------------------------------------------------------------------
+int cdh_fill(int64_t n, double value, double *x, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(n >= 0 && n < INT32_MAX && (x || n == 0), "cdh_fill: bad arguments");
+      if (n == 0) return;
+      hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), (int)n,
+                         value, x);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_set_list(int64_t n, const int *list, double value, double *x, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(n >= 0 && n < INT32_MAX && ((list && x) || n == 0), "cdh_set_list: bad arguments");
+      if (n == 0) return;
+      hipLaunchKernelGGL(k_set_list_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream),
+                         (int)n, list, value, x);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_integrate(int64_t n, double dt, double *Vm, const double *dVmR, const double *dVmD,
+                  const double *iStim, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(n >= 0 && n < INT32_MAX && ((Vm && dVmR && dVmD && iStim) || n == 0),
+                  "cdh_integrate: bad arguments");
+      if (n == 0) return;
+      hipLaunchKernelGGL(k_integrate_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream),
+                         (int)n, dt, Vm, dVmR, dVmD, iStim);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_activation_sensor_create(int64_t n, double threshold, cdh_sensor **out)
+{
+   return guard([&] {
+      CDH_REQUIRE(out && n >= 0 && n < INT32_MAX, "cdh_activation_sensor_create: bad arguments");
+      auto s = std::make_unique<cdh_sensor>();
+      s->n = n;
+      s->threshold = threshold;
+      const size_t m = n ? n : 1;
+      std::vector<double> neg(m, -1.0);
+      s->tact0.alloc(m);
+      s->tact0.upload(neg.data(), m);
+      s->tact.alloc(m);
+      s->tact.upload(neg.data(), m);
+      s->trec.alloc(m);
+      s->trec.upload(neg.data(), m);
+      s->active.alloc(m);
+      s->active.zero();
+      s->activated.alloc(m);
+      s->activated.zero();
+      *out = s.release();
+   });
+}
+
+int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(s && (Vm || s->n == 0), "cdh_activation_sensor_eval: bad arguments");
+      if (s->n == 0) return;
+      SensorArgs a;
+      a.n = (int)s->n;
+      a.enabled = 1;
+      a.first_eval = s->evaluated == 0;
+      a.time = time;
+      a.threshold = s->threshold;
+      a.tact0 = s->tact0.p;
+      a.tact = s->tact.p;
+      a.trec = s->trec.p;
+      a.active = s->active.p;
+      a.activated = s->activated.p;
+      hipLaunchKernelGGL(k_sensor_f64, dim3(grid_for(s->n, kBlock)), dim3(kBlock), 0, S(stream), a,
+                         Vm);
+      CDH_LAUNCH_CHECK();
+      s->evaluated = 1;
+   });
+}
+
+int cdh_activation_sensor_get(cdh_sensor *s, double *tact0, double *tact, double *trec)
+{
+   return guard([&] {
+      CDH_REQUIRE(s, "cdh_activation_sensor_get: null argument");
+      CDH_HIP(hipDeviceSynchronize());
+      if (tact0) s->tact0.download(tact0, s->n);
+      if (tact) s->tact.download(tact, s->n);
+      if (trec) s->trec.download(trec, s->n);
+   });
+}
+
+int cdh_activation_sensor_destroy(cdh_sensor *s)
+{
+   delete s;
+   return CDH_OK;
+}
+
+}  // extern "C"

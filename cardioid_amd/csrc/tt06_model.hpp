@@ -1,0 +1,90 @@
+// tt06_model.hpp -- TT06Dev model data shared by host and device code.
+//
+// State layout, constants, fit layout and cell-type parameters of the
+// TT06Dev reaction (configuration B: rational fits at tolerance 1e-4 with
+// mod = 1, "v2" gate kernels, generic non-gate update).
+//
+// References (reference tree):
+//   state order           elec/TT06Func.h:10-14
+//   constants c2..c44, f* elec/TT06NonGates.c:11-251
+//   fit order             elec/TT06Dev_Reaction.cc:206-207, :218-219
+//   fit orders (l, m)     hard-coded in elec/TT06GatesSimd.c (per gate), and
+//                         those of the fit file test/tests/multipleReaction/fit.data
+//   cell types            elec/TT06Func.cc:421-612
+#pragma once
+
+#include <cstdint>
+
+namespace cdh {
+
+enum StateSlot : int {
+   S_CA_I = 0, S_DVK_I, S_NA_I, S_CA_SS, S_CA_SR, S_R_PRIME, S_FCASS,
+   S_M, S_H, S_J, S_XR1, S_XR2, S_XS, S_R, S_D, S_F, S_F2, S_JL, S_S,
+   N_STATE
+};
+
+// Non-gate constants, derived on the host from the TT06 physical parameters.
+struct TT06Const {
+   double c2, c3, c4, c5, c6, c7, c8, c9, c11, c12, c13, c14, c15, c16, c17,
+          c18, c19, c20, c21, c22, c23, c24, c25, c26, c27, c28, c29, c30,
+          c31, c32, c33, c34, c36, c40, c43, c44;
+   double exp_coef[17];   // 1/i!, the m-gate Rush-Larsen series
+};
+
+// Fit orders (l, m) in fit order; the kernels are specialised on them.
+struct FitOrder { int l, m; };
+constexpr FitOrder kFitOrder[33] = {
+   {1, 10}, {1, 6}, {1, 8}, {1, 11}, {1, 9}, {5, 8}, {11, 7},          // fv0..fv6
+   {10, 5}, {1, 18},   // m
+   {7, 9}, {11, 11},   // h
+   {7, 9}, {1, 13},    // j
+   {5, 8}, {1, 13},    // Xr1
+   {1, 10}, {1, 10},   // Xr2
+   {5, 5}, {6, 9},     // Xs
+   {5, 8}, {5, 7},     // r
+   {5, 7}, {7, 10},    // d
+   {5, 8}, {13, 11},   // f
+   {8, 5}, {11, 12},   // f2
+   {6, 12}, {1, 1},    // jL
+   {7, 8}, {6, 7},     // s0
+   {7, 8}, {11, 11},   // s1
+};
+
+constexpr int fit_offset(int i)
+{
+   int o = 0;
+   for (int k = 0; k < i; k++) o += kFitOrder[k].l + kFitOrder[k].m;
+   return o;
+}
+constexpr int kFitCoefTotal = fit_offset(33);   // 468
+static_assert(kFitCoefTotal == 468, "fit layout");
+
+// Per-cell-type parameters (a reaction object's CellTypeParms).
+struct CellType {
+   int s_switch;
+   double P_NaK, g_Ks, g_Kr, g_to, g_NaL;
+   double Vm0;
+   double state0[N_STATE];   // K_i (not dVK_i) in slot S_DVK_I
+};
+
+constexpr int kMaxCellTypes = 8;
+
+// per-cell flag bits (see tt06_f64.hip): the reference's v2 gate loops stop
+// at 4*floor(n/4) of a reaction group, and its f-gate main loop updates
+// lanes 0..3 of each 12-cell group with a different rounding order.
+constexpr uint8_t kFlagGatesFixed = 1;
+constexpr uint8_t kFlagFGateSub = 2;
+
+inline uint8_t gate_flags(int64_t i, int64_t n)
+{
+   uint8_t f = 0;
+   if (i >= 4 * (n / 4)) f |= kFlagGatesFixed;
+   if (i < 12 * (n / 12) && (i % 12) < 4) f |= kFlagFGateSub;
+   return f;
+}
+
+// host-side helpers (tt06_host.cpp)
+void tt06_constants(TT06Const *k);
+bool tt06_standard_celltype(const char *name, CellType *out);
+
+}  // namespace cdh

@@ -1,0 +1,263 @@
+"""ctypes binding of libcardioid_hip.so (the C ABI in include/cardioid_hip.h).
+
+This module fails loudly when the HIP library is missing: there is no CPU
+fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .fitfile import DEFAULT_FIT_FILE, FitSet, read_fit_file
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcardioid_hip.so")
+
+NSTATE = 19
+MODE_FP64_UNFUSED = 0
+MODE_FP64_FUSED = 1
+MODES = {"fp64_unfused": MODE_FP64_UNFUSED, "fp64_fused": MODE_FP64_FUSED}
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+i64p = C.POINTER(C.c_int64)
+u8p = C.POINTER(C.c_uint8)
+
+
+class CellType(C.Structure):
+    _fields_ = [("s_switch", C.c_int), ("P_NaK", C.c_double), ("g_Ks", C.c_double),
+                ("g_Kr", C.c_double), ("g_to", C.c_double), ("g_NaL", C.c_double),
+                ("Vm0", C.c_double), ("state0", C.c_double * NSTATE)]
+
+
+class TT06Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("tolerance", C.c_double), ("mod", C.c_int),
+                ("fastGate", C.c_int), ("fastNonGate", C.c_int), ("fit_l", ip), ("fit_m", ip),
+                ("fit_coef", dp), ("celltype", CellType), ("s_kr", C.c_double),
+                ("s_cal", C.c_double), ("s_na", C.c_double)]
+
+
+class FGRParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double),
+                ("dy", C.c_double), ("dz", C.c_double), ("diffusionScale", C.c_double),
+                ("nLocal", C.c_int64), ("nRemote", C.c_int64), ("gid", i64p), ("cellType", ip),
+                ("sigma", dp)]
+
+
+class BoxStim(C.Structure):
+    _fields_ = [("xMin", C.c_int), ("xMax", C.c_int), ("yMin", C.c_int), ("yMax", C.c_int),
+                ("zMin", C.c_int), ("zMax", C.c_int), ("t0", C.c_double), ("tf", C.c_double),
+                ("period", C.c_double), ("duration", C.c_double), ("amplitude", C.c_double),
+                ("phase", C.c_double)]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double),
+                ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+                ("diffusionScale", C.c_double), ("ncell", C.c_int64), ("gid", i64p),
+                ("celltype", u8p), ("sigma", dp), ("ncelltype", C.c_int),
+                ("celltypes", C.POINTER(CellType)), ("scales", dp), ("fit_l", ip), ("fit_m", ip),
+                ("fit_coef", dp), ("threshold", C.c_double), ("mode", C.c_int),
+                ("gate_tail_rule", C.c_int), ("sensors", C.c_int)]
+
+
+EXPORTS = [
+    "cdh_last_error", "cdh_version", "cdh_celltype_standard",
+    "cdh_tt06_create", "cdh_tt06_destroy", "cdh_tt06_calc", "cdh_tt06_update_nongate",
+    "cdh_tt06_update_gate", "cdh_tt06_initialize_membrane_voltage", "cdh_tt06_checkpoint_field",
+    "cdh_tt06_var_handle", "cdh_tt06_get_value", "cdh_tt06_set_value", "cdh_tt06_get_state",
+    "cdh_tt06_set_state",
+    "cdh_fgr_create", "cdh_fgr_destroy", "cdh_fgr_update_local_voltage",
+    "cdh_fgr_update_remote_voltage", "cdh_fgr_calc", "cdh_fgr_diffusion_scale",
+    "cdh_fill", "cdh_set_list", "cdh_integrate",
+    "cdh_activation_sensor_create", "cdh_activation_sensor_eval", "cdh_activation_sensor_get",
+    "cdh_activation_sensor_destroy",
+    "cdh_sim_create", "cdh_sim_destroy", "cdh_sim_run", "cdh_sim_sync", "cdh_sim_run_timed",
+    "cdh_sim_get_vm", "cdh_sim_get_state", "cdh_sim_set_state", "cdh_sim_get_activation",
+    "cdh_sim_time", "cdh_sim_loop", "cdh_sim_info",
+]
+
+_lib = None
+
+
+class CardioidHipError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CardioidHipError(
+                f"{LIB_PATH} is missing: build it with `python -m cardioid_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.cdh_last_error.restype = C.c_char_p
+        L.cdh_fgr_diffusion_scale.restype = C.c_double
+        L.cdh_sim_time.restype = C.c_double
+        L.cdh_sim_time.argtypes = [C.c_void_p]
+        L.cdh_sim_loop.argtypes = [C.c_void_p]
+        for name in ("cdh_tt06_create",):
+            getattr(L, name).argtypes = [C.POINTER(TT06Params), C.c_int, C.POINTER(C.c_void_p)]
+        L.cdh_tt06_calc.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cdh_tt06_update_nongate.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int,
+                                              C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cdh_tt06_update_gate.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int,
+                                           C.c_void_p, C.c_void_p]
+        L.cdh_tt06_initialize_membrane_voltage.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
+                                                           C.c_void_p, C.c_void_p]
+        L.cdh_tt06_get_state.argtypes = [C.c_void_p, dp]
+        L.cdh_tt06_set_state.argtypes = [C.c_void_p, dp]
+        L.cdh_tt06_get_value.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.cdh_tt06_set_value.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+        L.cdh_tt06_destroy.argtypes = [C.c_void_p]
+        L.cdh_tt06_var_handle.argtypes = [C.c_char_p]
+        L.cdh_tt06_checkpoint_field.argtypes = [C.c_int, C.POINTER(C.c_char_p),
+                                                C.POINTER(C.c_char_p), ip]
+        L.cdh_fgr_create.argtypes = [C.POINTER(FGRParams), C.POINTER(C.c_void_p)]
+        L.cdh_fgr_destroy.argtypes = [C.c_void_p]
+        L.cdh_fgr_update_local_voltage.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cdh_fgr_update_remote_voltage.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cdh_fgr_calc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cdh_fgr_diffusion_scale.argtypes = [C.c_void_p]
+        L.cdh_fill.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p]
+        L.cdh_set_list.argtypes = [C.c_int64, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+        L.cdh_integrate.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
+        L.cdh_activation_sensor_create.argtypes = [C.c_int64, C.c_double, C.POINTER(C.c_void_p)]
+        L.cdh_activation_sensor_eval.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+        L.cdh_activation_sensor_get.argtypes = [C.c_void_p, dp, dp, dp]
+        L.cdh_activation_sensor_destroy.argtypes = [C.c_void_p]
+        L.cdh_sim_create.argtypes = [C.POINTER(SimParams), C.c_int, C.POINTER(BoxStim),
+                                     C.POINTER(C.c_void_p)]
+        L.cdh_sim_destroy.argtypes = [C.c_void_p]
+        L.cdh_sim_run.argtypes = [C.c_void_p, C.c_int]
+        L.cdh_sim_sync.argtypes = [C.c_void_p]
+        L.cdh_sim_run_timed.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cdh_sim_get_vm.argtypes = [C.c_void_p, dp]
+        L.cdh_sim_get_state.argtypes = [C.c_void_p, dp]
+        L.cdh_sim_set_state.argtypes = [C.c_void_p, dp, dp]
+        L.cdh_sim_get_activation.argtypes = [C.c_void_p, dp, dp, dp]
+        L.cdh_sim_info.argtypes = [C.c_void_p, i64p, ip, ip, ip]
+        L.cdh_celltype_standard.argtypes = [C.c_char_p, C.POINTER(CellType)]
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().cdh_last_error().decode(errors="replace")
+        raise CardioidHipError(f"{what}: error {rc}: {msg}")
+
+
+def ptr(a, t=dp):
+    return a.ctypes.data_as(t)
+
+
+def standard_celltype(name: str) -> CellType:
+    ct = CellType()
+    check(lib().cdh_celltype_standard(name.encode(), C.byref(ct)), "cdh_celltype_standard")
+    return ct
+
+
+class Simulation:
+    """Monodomain engine over the C ABI (cdh_sim_*).
+
+    anatomy: an Anatomy from cardioid_amd.workloads (gid, celltype index,
+    conductivity per tissue cell); stimuli: list of BoxStim.
+    """
+
+    def __init__(self, anatomy, celltypes: Sequence[str], dt: float, diffusion_scale: float,
+                 stimuli=(), mode: str = "fp64_fused", threshold: float = -40.0,
+                 fits: Optional[FitSet] = None, scales: Optional[np.ndarray] = None,
+                 gate_tail_rule: bool = True, sensors: bool = True):
+        L = lib()
+        self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
+        fl, fm, fc = self.fits.packed()
+        self._keep = [fl, fm, fc]
+        cts = (CellType * len(celltypes))(*[standard_celltype(n) for n in celltypes])
+        self._keep.append(cts)
+        self.ncell = int(anatomy.gid.size)
+        gid = np.ascontiguousarray(anatomy.gid, dtype=np.int64)
+        ctype = np.ascontiguousarray(anatomy.celltype, dtype=np.uint8)
+        sigma = np.ascontiguousarray(anatomy.sigma, dtype=np.float64).reshape(-1)
+        self._keep += [gid, ctype, sigma]
+        sc = None
+        if scales is not None:
+            sc = np.ascontiguousarray(scales, dtype=np.float64).reshape(-1)
+            self._keep.append(sc)
+        p = SimParams(nx=anatomy.nx, ny=anatomy.ny, nz=anatomy.nz, dx=anatomy.dx, dy=anatomy.dy,
+                      dz=anatomy.dz, dt=dt, diffusionScale=diffusion_scale, ncell=self.ncell,
+                      gid=ptr(gid, i64p), celltype=ptr(ctype, u8p), sigma=ptr(sigma),
+                      ncelltype=len(celltypes), celltypes=cts,
+                      scales=ptr(sc) if sc is not None else None,
+                      fit_l=ptr(fl, ip), fit_m=ptr(fm, ip), fit_coef=ptr(fc), threshold=threshold,
+                      mode=MODES[mode], gate_tail_rule=int(gate_tail_rule), sensors=int(sensors))
+        st = (BoxStim * max(1, len(stimuli)))(*stimuli)
+        h = C.c_void_p()
+        check(L.cdh_sim_create(C.byref(p), len(stimuli), st, C.byref(h)), "cdh_sim_create")
+        self.h = h
+        self.dt = dt
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdh_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps: int):
+        check(lib().cdh_sim_run(self.h, int(nsteps)), "cdh_sim_run")
+
+    def run_timed(self, nsteps: int) -> float:
+        ms = C.c_double()
+        check(lib().cdh_sim_run_timed(self.h, int(nsteps), C.byref(ms)), "cdh_sim_run_timed")
+        return ms.value
+
+    def sync(self):
+        check(lib().cdh_sim_sync(self.h), "cdh_sim_sync")
+
+    def vm(self) -> np.ndarray:
+        out = np.zeros(self.ncell)
+        check(lib().cdh_sim_get_vm(self.h, ptr(out)), "cdh_sim_get_vm")
+        return out
+
+    def state(self) -> np.ndarray:
+        out = np.zeros(NSTATE * self.ncell)
+        check(lib().cdh_sim_get_state(self.h, ptr(out)), "cdh_sim_get_state")
+        return out.reshape(NSTATE, self.ncell)
+
+    def set_state(self, state=None, vm=None):
+        s = None if state is None else np.ascontiguousarray(state, dtype=np.float64).reshape(-1)
+        v = None if vm is None else np.ascontiguousarray(vm, dtype=np.float64)
+        check(lib().cdh_sim_set_state(self.h, ptr(s) if s is not None else None,
+                                      ptr(v) if v is not None else None), "cdh_sim_set_state")
+
+    def activation(self):
+        a0, a, r = (np.zeros(self.ncell) for _ in range(3))
+        check(lib().cdh_sim_get_activation(self.h, ptr(a0), ptr(a), ptr(r)),
+              "cdh_sim_get_activation")
+        return a0, a, r
+
+    @property
+    def time(self) -> float:
+        return lib().cdh_sim_time(self.h)
+
+    @property
+    def loop(self) -> int:
+        return lib().cdh_sim_loop(self.h)
+
+    def info(self):
+        b = C.c_int64()
+        bx, by, bz = C.c_int(), C.c_int(), C.c_int()
+        check(lib().cdh_sim_info(self.h, C.byref(b), C.byref(bx), C.byref(by), C.byref(bz)),
+              "cdh_sim_info")
+        return dict(device_bytes=b.value, box=(bx.value, by.value, bz.value))

@@ -1,0 +1,166 @@
+"""Synthetic anatomies, conductivities and stimuli for the benchmark and
+parity workloads (seeded; no input files).
+
+Units follow Cardioid's internal ones (elec/cardioid.cc:43-63): mm, ms, mV,
+conductivity in S/mm, diffusionScale in mm^3/mF.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Tuple
+
+import numpy as np
+
+# Niederer-deck diffusion scale (test/tests/Niederer/object.data, fgr DIFFUSION)
+DIFFUSION_SCALE = 714.2857143
+SIGMA_L = 0.1334e-3    # S/mm  (0.1334 mS/mm)
+SIGMA_T = 0.0176e-3    # S/mm  (0.0176 mS/mm)
+CELLTYPES_EMI = ["endoCellML", "midCellML", "epiCellML"]
+
+
+@dataclass
+class Anatomy:
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    gid: np.ndarray        # int64, ascending; gid = x + nx*(y + ny*z)
+    celltype: np.ndarray   # uint8 index into the cell-type list
+    sigma: np.ndarray      # (n, 6): a11 a12 a13 a22 a23 a33, S/mm
+
+    @property
+    def ncell(self) -> int:
+        return int(self.gid.size)
+
+    def tuples(self):
+        g = self.gid
+        x = g % self.nx
+        y = (g // self.nx) % self.ny
+        z = g // (self.nx * self.ny)
+        return x, y, z
+
+
+def fibre_tensor(f: np.ndarray, sl: float = SIGMA_L, st: float = SIGMA_T) -> np.ndarray:
+    """sigma = st*I + (sl - st) f f^T for unit fibre vectors f (n, 3), the
+    tensor FibreConductivity builds from the fibre angles
+    (elec/FibreConductivity.cc:27-62)."""
+    d = sl - st
+    out = np.empty((f.shape[0], 6))
+    out[:, 0] = st + d * f[:, 0] * f[:, 0]
+    out[:, 1] = d * f[:, 0] * f[:, 1]
+    out[:, 2] = d * f[:, 0] * f[:, 2]
+    out[:, 3] = st + d * f[:, 1] * f[:, 1]
+    out[:, 4] = d * f[:, 1] * f[:, 2]
+    out[:, 5] = st + d * f[:, 2] * f[:, 2]
+    return out
+
+
+def brick(nx: int, ny: int, nz: int, dx: float = 0.2, celltype: int = 0,
+          sigma=(SIGMA_L, SIGMA_T, SIGMA_T)) -> Anatomy:
+    """All-tissue brick with uniform diagonal conductivity (the `uniform`
+    CONDUCTIVITY method, elec/setConductivity.cc:234-247)."""
+    n = nx * ny * nz
+    gid = np.arange(n, dtype=np.int64)
+    s = np.zeros((n, 6))
+    s[:, 0], s[:, 3], s[:, 5] = sigma
+    return Anatomy(nx, ny, nz, dx, dx, dx, gid, np.full(n, celltype, np.uint8), s)
+
+
+def ellipsoid_shell(nx: int = 256, ny: int = 256, nz: int = 512, dx: float = 0.2, seed: int = 0,
+                    occupancy: float = 0.60, wall: float = 0.45) -> Tuple[Anatomy, np.ndarray]:
+    """Truncated ellipsoidal shell in the (nx, ny, nz) box.
+
+    The outer surface is the ellipsoid rho = 1 with semi-axes s*(nx, ny, nz)/2
+    (clipped by the box), the inner one rho = 1 - wall; s is chosen so the
+    tissue fills `occupancy` of the box.  Cell types by normalised transmural
+    depth d (0 at the outer, epicardial surface, 1 at the inner one): epi for
+    d < 1/3, mid for 1/3 <= d < 2/3, endo beyond.  Fibres lie in the surface
+    tangent plane at helix angle -60 deg (epi) to +60 deg (endo), linear in d,
+    with a seeded +-2 deg jitter.  Returns the anatomy and the depth d.
+    """
+    rng = np.random.default_rng(seed)
+    cx, cy, cz = (nx - 1) / 2, (ny - 1) / 2, (nz - 1) / 2
+
+    def occ(s):
+        xs = (np.arange(nx) - cx) / (s * nx / 2)
+        ys = (np.arange(ny) - cy) / (s * ny / 2)
+        zs = (np.arange(nz) - cz) / (s * nz / 2)
+        r2 = xs[:, None, None] ** 2 + ys[None, :, None] ** 2  # (x, y, 1)
+        frac = 0.0
+        for k in range(0, nz, max(1, nz // 64)):
+            rr = np.sqrt(r2[:, :, 0] + zs[k] ** 2)
+            frac += np.mean((rr <= 1.0) & (rr >= 1.0 - wall))
+        return frac / len(range(0, nz, max(1, nz // 64)))
+
+    lo, hi = 0.5, 2.0
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if occ(mid) < occupancy:
+            lo = mid
+        else:
+            hi = mid
+    s = 0.5 * (lo + hi)
+    ax, ay, az = s * nx / 2, s * ny / 2, s * nz / 2
+
+    gids, depth, fib = [], [], []
+    xs = np.arange(nx)
+    X, Y = np.meshgrid(xs, np.arange(ny), indexing="ij")   # (x, y)
+    ex, ey = (X - cx) / ax, (Y - cy) / ay
+    for z in range(nz):
+        ez = (z - cz) / az
+        rho = np.sqrt(ex ** 2 + ey ** 2 + ez ** 2)
+        m = (rho <= 1.0) & (rho >= 1.0 - wall)
+        if not m.any():
+            continue
+        xx, yy = X[m], Y[m]
+        d = (1.0 - rho[m]) / wall
+        # surface frame: outward normal of the ellipsoid, circumferential
+        # direction around z, longitudinal direction in the tangent plane
+        nrm = np.stack([ex[m] / ax, ey[m] / ay, np.full(xx.size, ez / az)], -1)
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True) + 1e-30
+        circ = np.stack([-(yy - cy), (xx - cx), np.zeros(xx.size)], -1).astype(np.float64)
+        cn = np.linalg.norm(circ, axis=1, keepdims=True)
+        circ = np.where(cn > 1e-9, circ / np.maximum(cn, 1e-30), np.array([1.0, 0.0, 0.0]))
+        lon = np.cross(nrm, circ)
+        lon /= np.linalg.norm(lon, axis=1, keepdims=True) + 1e-30
+        ang = np.deg2rad(-60.0 + 120.0 * d + rng.uniform(-2, 2, d.size))
+        f = np.cos(ang)[:, None] * circ + np.sin(ang)[:, None] * lon
+        f /= np.linalg.norm(f, axis=1, keepdims=True)
+        gids.append(xx + nx * (yy + ny * z))
+        depth.append(d)
+        fib.append(f)
+    gid = np.concatenate(gids).astype(np.int64)
+    order = np.argsort(gid, kind="stable")
+    gid = gid[order]
+    depth = np.concatenate(depth)[order]
+    f = np.concatenate(fib)[order]
+    ctype = np.where(depth < 1 / 3, 2, np.where(depth < 2 / 3, 1, 0)).astype(np.uint8)
+    return Anatomy(nx, ny, nz, dx, dx, dx, gid, ctype, fibre_tensor(f)), depth
+
+
+def box_stimulus(xMin=-1, xMax=None, yMin=-1, yMax=None, zMin=-1, zMax=None, nx=0, ny=0, nz=0,
+                 vStim=-52.0, tStart=0.0, duration=1.0, period=1000.0, t0=-1000.0, tf=1e30):
+    """A box STIMULUS with a periodic pulse and the factory's defaults
+    (elec/stimulusFactory.cc:24-67, :93-118): amplitude = -vStim."""
+    from .hip import BoxStim
+    return BoxStim(xMin=xMin, xMax=nx if xMax is None else xMax, yMin=yMin,
+                   yMax=ny if yMax is None else yMax, zMin=zMin,
+                   zMax=nz if zMax is None else zMax, t0=t0, tf=tf, period=period,
+                   duration=duration, amplitude=-vStim, phase=tStart)
+
+
+def endocardial_point_stimuli(anat: Anatomy, depth: np.ndarray, n: int = 8, seed: int = 0,
+                              half: int = 2, vStim: float = -52.0, duration: float = 1.0):
+    """n seeded sites on the inner (endocardial) surface; each a small box
+    stimulus of (2*half-1)^3 grid points around the site."""
+    rng = np.random.default_rng(seed)
+    cand = np.nonzero(depth > 0.97)[0]
+    pick = rng.choice(cand, size=n, replace=False)
+    x, y, z = anat.tuples()
+    out: List = []
+    for c in pick:
+        out.append(box_stimulus(x[c] - half, x[c] + half, y[c] - half, y[c] + half,
+                                z[c] - half, z[c] + half, vStim=vStim, duration=duration))
+    return out

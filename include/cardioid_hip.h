@@ -1,0 +1,217 @@
+/*
+ * cardioid_hip.h -- C ABI of the MI355X (gfx950) implementation of
+ * Cardioid's electrics hot path: the TT06Dev reaction and the FGR 19-point
+ * diffusion stencil, plus the loop pieces that drive them.
+ *
+ * All pointers named as device arrays (Vm, dVm, iStim, index, ...) are HIP
+ * device pointers, i.e. what lazy_array::useOn(GPU) hands out in the
+ * reference (elec/lazy_array.hh:204-418).  `stream` is a hipStream_t passed
+ * as void* (NULL = the default stream).  Host arrays are plain host memory.
+ *
+ * Every function returns 0 on success or a CDH_ERR_* code; the message of
+ * the last failure on the calling thread is cdh_last_error().  Where the
+ * reference asserts on bad input (reactionFactory.cc:70-78,
+ * diffusionFactory.cc:65-67) these return CDH_ERR_ARG instead.
+ *
+ * Reference interfaces replaced, by group:
+ *   cdh_tt06_*  -> class Reaction (elec/Reaction.hh:10-43) as implemented by
+ *                  TT06Dev_Reaction (elec/TT06Dev_Reaction.cc:83-664) and made
+ *                  by REACTION_FACTORY(TT06Dev) (:583-660)
+ *   cdh_fgr_*   -> class Diffusion (elec/Diffusion.hh:19-39) as implemented by
+ *                  FGRDiffusionOMP (elec/FGRDiffusionOMP.cc:14-265) and chosen
+ *                  in diffusionFactory (elec/diffusionFactory.cc:38-103)
+ *   cdh_fill / cdh_set_list / cdh_integrate -> the DEVICE_PARALLEL_FORALL
+ *                  bodies of simulationLoop (elec/simulationLoop.cc:186-193,
+ *                  :221-227) and BoxStimulus::subClassStim (elec/BoxStimulus.cc:53-64)
+ *   cdh_activation_sensor_* -> ActivationTimeSensor::eval
+ *                  (elec/ActivationTimeSensor.cc:99-111) and
+ *                  ActivationAndRecoverySensor::eval (elec/ActivationAndRecoverySensor.cc:76-92)
+ *   cdh_sim_*   -> simulationLoop (elec/simulationLoop.cc:150-243) with the
+ *                  reaction, diffusion, stimuli and sensors above, device resident
+ */
+#ifndef CARDIOID_HIP_H
+#define CARDIOID_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CDH_OK 0
+#define CDH_ERR_ARG 1
+#define CDH_ERR_HIP 2
+#define CDH_ERR_UNSUPPORTED 3
+#define CDH_ERR_STATE 4
+
+#define CDH_NSTATE 19
+#define CDH_NFIT 33
+
+const char *cdh_last_error(void);
+int cdh_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Cell types (CellTypeParmsFull, elec/TT06Func.hh:49-56).  state0 is in the */
+/* TT06_DEV_STATE_INDEX order (elec/TT06Func.h:10-14) with K_i (not dVK_i)   */
+/* in slot 1, as getStandardCellTypes fills it.                              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+   int s_switch;
+   double P_NaK, g_Ks, g_Kr, g_to, g_NaL;
+   double Vm0;
+   double state0[CDH_NSTATE];
+} cdh_celltype;
+
+/* TT06Func::getStandardCellTypes (elec/TT06Func.cc:421-612): endoCellML,
+ * midCellML, epiCellML, endoRRG, midRRG, epiRRG. */
+int cdh_celltype_standard(const char *name, cdh_celltype *out);
+
+/* ------------------------------------------------------------------------ */
+/* Reaction: TT06Dev                                                         */
+/* ------------------------------------------------------------------------ */
+typedef struct cdh_tt06 cdh_tt06;
+
+typedef struct {
+   double dt;                  /* time step the fits are scaled by (ms) */
+   double tolerance;           /* REACTION keyword; must be 1e-4 */
+   int mod;                    /* REACTION keyword (jhTauSmooth); must be 1 */
+   int fastGate;               /* must be >= 1: the v2 gate kernels */
+   int fastNonGate;            /* must be 0: the generic non-gate update */
+   const int *fit_l;           /* 33 fits in the order of the fitFile's      */
+   const int *fit_m;           /* `functions` list; coefficients packed, l+m */
+   const double *fit_coef;     /* per fit (elec/pade.hh:4-26 layout)          */
+   cdh_celltype celltype;      /* from cellTypeName / CELLTYPE clone          */
+   double s_kr, s_cal, s_na;   /* conductance scales on g_Kr, g_CaL, g_Na (1 = reference) */
+} cdh_tt06_params;
+
+/* REACTION_FACTORY(TT06Dev) + TT06Dev_Reaction ctor
+ * (elec/TT06Dev_Reaction.cc:583-660, :83-157).  State is allocated on the
+ * device and initialised to the cell type's rest state (:180-200). */
+int cdh_tt06_create(const cdh_tt06_params *p, int numPoints, cdh_tt06 **out);
+int cdh_tt06_destroy(cdh_tt06 *r);
+
+/* Reaction::calc (elec/Reaction.hh:15-17; TT06Dev :393-407): for i < n,
+ * reads Vm[indexArray[i]], writes dVm[indexArray[i]] = -Iion, advances the
+ * 19 states by dt.  indexArray may be NULL (identity).  iStim is ignored,
+ * as TT06Dev ignores it. */
+int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const double *Vm,
+                  const double *iStim, double *dVm, void *stream);
+/* Reaction::updateNonGate / updateGate (pdr loop; TT06Dev :408-453) */
+int cdh_tt06_update_nongate(cdh_tt06 *r, double dt, const int *indexArray, int n,
+                            const double *Vm, double *dVR, void *stream);
+int cdh_tt06_update_gate(cdh_tt06 *r, double dt, const int *indexArray, int n,
+                         const double *Vm, void *stream);
+/* Reaction::initializeMembraneVoltage (TT06Dev :455-460) */
+int cdh_tt06_initialize_membrane_voltage(cdh_tt06 *r, const int *indexArray, int n,
+                                         double *Vm, void *stream);
+/* checkpoint accessors: getCheckpointInfo / getVarHandle / getValue / setValue
+ * (TT06Dev :462-555).  Field i in 0..18 -> name, unit, handle. */
+int cdh_tt06_checkpoint_field(int i, const char **name, const char **unit, int *handle);
+int cdh_tt06_var_handle(const char *name);      /* -1 if unknown */
+int cdh_tt06_get_value(cdh_tt06 *r, int iCell, int handle, double *value);
+int cdh_tt06_set_value(cdh_tt06 *r, int iCell, int handle, double value);
+/* bulk state transfer, host array [19][numPoints] */
+int cdh_tt06_get_state(cdh_tt06 *r, double *host);
+int cdh_tt06_set_state(cdh_tt06 *r, const double *host);
+
+/* ------------------------------------------------------------------------ */
+/* Diffusion: FGR                                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct cdh_fgr cdh_fgr;
+
+typedef struct {
+   int nx, ny, nz;             /* anatomy grid; gid = x + nx*(y + ny*z) */
+   double dx, dy, dz;          /* mm */
+   double diffusionScale;      /* mm^3/mF */
+   int64_t nLocal, nRemote;
+   const int64_t *gid;         /* nLocal + nRemote cells, local first */
+   const int *cellType;        /* tissue iff cellType > 0 (elec/FGRUtils.hh:42) */
+   const double *sigma;        /* 6 per cell: a11 a12 a13 a22 a23 a33 (S/mm) */
+} cdh_fgr_params;
+
+/* FGRDiffusionOMP ctor: bounding box of the local cells padded by one
+ * (elec/DiffusionUtils.cc:19-66), weights (elec/FGRDiffusionOMP.cc:166-265)
+ * in float with a double centre weight, as the CMake build has them. */
+int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out);
+int cdh_fgr_destroy(cdh_fgr *d);
+/* Diffusion::updateLocalVoltage / updateRemoteVoltage: scatter into the box */
+int cdh_fgr_update_local_voltage(cdh_fgr *d, const double *VmLocal, void *stream);
+int cdh_fgr_update_remote_voltage(cdh_fgr *d, const double *VmRemote, void *stream);
+/* Diffusion::calc, omp-loop contract: dVm[i] = scale*(stencil) for i < nLocal */
+int cdh_fgr_calc(cdh_fgr *d, double *dVm, void *stream);
+double cdh_fgr_diffusion_scale(const cdh_fgr *d);
+
+/* ------------------------------------------------------------------------ */
+/* Loop pieces                                                               */
+/* ------------------------------------------------------------------------ */
+int cdh_fill(int64_t n, double value, double *x, void *stream);
+int cdh_set_list(int64_t n, const int *list, double value, double *x, void *stream);
+int cdh_integrate(int64_t n, double dt, double *Vm, const double *dVmR, const double *dVmD,
+                  const double *iStim, void *stream);
+
+typedef struct cdh_sensor cdh_sensor;
+/* activation bookkeeping for n cells: first Vm > 0 (ActivationTimeSensor),
+ * first up-crossing of `threshold` and the first down-crossing after it
+ * (ActivationAndRecoverySensor) */
+int cdh_activation_sensor_create(int64_t n, double threshold, cdh_sensor **out);
+int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, void *stream);
+int cdh_activation_sensor_get(cdh_sensor *s, double *tact0, double *tact, double *trec);
+int cdh_activation_sensor_destroy(cdh_sensor *s);
+
+/* ------------------------------------------------------------------------ */
+/* Monodomain engine: simulationLoop with everything device resident         */
+/* ------------------------------------------------------------------------ */
+typedef struct cdh_sim cdh_sim;
+
+/* box stimulus with a periodic pulse (stimulusFactory.cc:24-67,
+ * BoxStimulus.cc:18-34, PeriodicPulse.cc:22-28): cells with xMin < x < xMax
+ * (and y, z likewise) get iStim = amplitude while t0 < time < tf and
+ * phase <= time mod period < phase + duration.  amplitude = -vStim. */
+typedef struct {
+   int xMin, xMax, yMin, yMax, zMin, zMax;
+   double t0, tf, period, duration, amplitude, phase;
+} cdh_box_stim;
+
+#define CDH_MODE_FP64_UNFUSED 0   /* reference-structured: reaction, scatter, stencil, integrate */
+#define CDH_MODE_FP64_FUSED 1     /* one launch per step, same arithmetic */
+
+typedef struct {
+   int nx, ny, nz;
+   double dx, dy, dz, dt, diffusionScale;
+   int64_t ncell;               /* tissue cells */
+   const int64_t *gid;          /* ncell */
+   const uint8_t *celltype;     /* ncell, index into celltypes */
+   const double *sigma;         /* 6*ncell */
+   int ncelltype;               /* <= 8 */
+   const cdh_celltype *celltypes;
+   const double *scales;        /* 3*ncell (s_kr, s_cal, s_na) or NULL */
+   const int *fit_l;
+   const int *fit_m;
+   const double *fit_coef;
+   double threshold;            /* activation threshold, mV (-40 default) */
+   int mode;                    /* CDH_MODE_* */
+   int gate_tail_rule;          /* 1: the v2 loops' per-group bounds (reference) */
+   int sensors;                 /* 1: record activation times */
+} cdh_sim_params;
+
+int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim, cdh_sim **out);
+int cdh_sim_destroy(cdh_sim *s);
+/* advances nsteps omp-loop steps; asynchronous on the engine's stream */
+int cdh_sim_run(cdh_sim *s, int nsteps);
+int cdh_sim_sync(cdh_sim *s);
+/* runs nsteps and returns the GPU time between events recorded on the
+ * engine's stream around them, in ms */
+int cdh_sim_run_timed(cdh_sim *s, int nsteps, double *ms);
+int cdh_sim_get_vm(cdh_sim *s, double *host);          /* ncell, input order */
+int cdh_sim_get_state(cdh_sim *s, double *host);       /* [19][ncell], input order */
+int cdh_sim_set_state(cdh_sim *s, const double *state, const double *Vm);
+int cdh_sim_get_activation(cdh_sim *s, double *tact0, double *tact, double *trec);
+double cdh_sim_time(const cdh_sim *s);
+int cdh_sim_loop(const cdh_sim *s);
+/* bytes of device memory held, and the box dimensions used */
+int cdh_sim_info(const cdh_sim *s, int64_t *device_bytes, int *bx, int *by, int *bz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

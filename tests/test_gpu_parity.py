@@ -1,0 +1,200 @@
+"""Parity of the HIP path (through the C ABI) with the oracle, on an MI355X.
+
+Bar (FP64 drop-in): per-cell Vm within 1e-9 relative to max(|Vm|, 1 mV)
+after 1000 steps of dt = 0.02 ms, and activation times identical; the
+reaction and stencil kernels alone agree to the last bit except where the
+device log() rounds differently from glibc's (then within 1e-14 relative).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from cardioid_amd import hip, workloads as W
+from cardioid_amd.fitfile import read_fit_file
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+FS = read_fit_file()
+DT = 0.02
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.cuda()
+
+
+def dp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def relerr(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    hip.lib()
+    yield
+
+
+@pytest.mark.parametrize("name,scales", [("endoCellML", (1.0, 1.0, 1.0)),
+                                         ("epiCellML", (1.0, 1.0, 1.0)),
+                                         ("midRRG", (0.3, 0.7, 0.5))])
+def test_reaction_plugin_matches_oracle(name, scales):
+    import oracle_lib as OL
+    say(f"reaction plugin {name} {scales}")
+    L = hip.lib()
+    n = 1003
+    rng = np.random.default_rng(7)
+    ct = OL.celltype(name)
+    st = np.zeros(19 * n)
+    vm = np.zeros(n)
+    OL.oracle().ora_tt06_init_state(C.byref(ct), n, OL.ptr(st), n, OL.ptr(vm))
+    S = st.reshape(19, n)
+    vm[:] = rng.uniform(-90, 40, n)
+    S[:] *= rng.uniform(0.8, 1.2, (19, n))
+    S[1] = 138.0 * rng.uniform(0.95, 1.05, n) / OL.oracle().ora_c9() + vm
+    l, m, c = FS.packed()
+    p = hip.TT06Params(dt=DT, tolerance=1e-4, mod=1, fastGate=1, fastNonGate=0,
+                       fit_l=hip.ptr(l, hip.ip), fit_m=hip.ptr(m, hip.ip), fit_coef=hip.ptr(c),
+                       celltype=hip.standard_celltype(name), s_kr=scales[0], s_cal=scales[1],
+                       s_na=scales[2])
+    h = C.c_void_p()
+    hip.check(L.cdh_tt06_create(C.byref(p), n, C.byref(h)), "create")
+    try:
+        hip.check(L.cdh_tt06_set_state(h, hip.ptr(st)), "set_state")
+        perm = rng.permutation(n).astype(np.int32)
+        Vfull = np.zeros(n)
+        Vfull[perm] = vm                      # Vm[index[i]] is cell i's voltage
+        d_idx, d_vm, d_dvm = dev(perm), dev(Vfull), dev(np.full(n, np.nan))
+        hip.check(L.cdh_tt06_calc(h, DT, dp(d_idx), n, dp(d_vm), None, dp(d_dvm), None), "calc")
+        torch.cuda.synchronize()
+        got_st = np.zeros(19 * n)
+        hip.check(L.cdh_tt06_get_state(h, hip.ptr(got_st)), "get_state")
+        got_dvm = d_dvm.cpu().numpy()[perm]
+    finally:
+        L.cdh_tt06_destroy(h)
+    ofs = OL.make_fitset(FS, DT)
+    sc = OL.OraScales(*scales)
+    dv = np.zeros(n)
+    OL.oracle().ora_tt06_nongate(C.byref(ofs), DT, C.byref(ct), C.byref(sc), n, OL.ptr(vm),
+                                 OL.ptr(st), n, OL.ptr(dv))
+    OL.oracle().ora_tt06_gates_v2(C.byref(ofs), ct.s_switch, n, OL.ptr(vm), OL.ptr(st), n)
+    e_dv = np.abs(got_dvm - dv) / np.maximum(np.abs(dv), 1e-300)
+    e_st = np.abs(got_st - st) / np.maximum(np.abs(st), 1e-300)
+    say(f"  bitwise dVm {np.mean(got_dvm == dv):.4f} state {np.mean(got_st == st):.4f} "
+        f"max rel {e_dv.max():.2e} {e_st.max():.2e}")
+    assert e_dv.max() < 1e-13 and e_st.max() < 1e-13
+    assert np.mean(got_st == st) > 0.99
+
+
+def test_fgr_plugin_matches_oracle_with_remote_cells():
+    import oracle_lib as OL
+    from test_oracle_fgr import apply, build, spd_tensor
+    say("fgr plugin")
+    L = hip.lib()
+    rng = np.random.default_rng(3)
+    X, Y, Z = 14, 12, 10
+    t = (rng.random((X, Y, Z)) < 0.75).astype(np.uint8)
+    s6 = spd_tensor(rng, X * Y * Z).reshape(X, Y, Z, 6) * 1e-3
+    # anatomy cells: local = tissue with x <= 6, remote = tissue at x == 7
+    cells = np.argwhere(t.astype(bool))
+    gid = cells[:, 0] + X * (cells[:, 1] + Y * cells[:, 2])
+    local = cells[:, 0] <= 6
+    remote = cells[:, 0] == 7
+    order = np.concatenate([np.nonzero(local)[0], np.nonzero(remote)[0]])
+    g = gid[order].astype(np.int64)
+    sig = s6[cells[order, 0], cells[order, 1], cells[order, 2]].reshape(-1).copy()
+    ctype = np.ones(order.size, np.int32)
+    nL, nR = int(local.sum()), int(remote.sum())
+    prm = hip.FGRParams(nx=X, ny=Y, nz=Z, dx=0.2, dy=0.2, dz=0.2, diffusionScale=W.DIFFUSION_SCALE,
+                        nLocal=nL, nRemote=nR, gid=hip.ptr(g, hip.i64p),
+                        cellType=hip.ptr(ctype, hip.ip), sigma=hip.ptr(sig))
+    h = C.c_void_p()
+    hip.check(L.cdh_fgr_create(C.byref(prm), C.byref(h)), "fgr create")
+    try:
+        vm = rng.uniform(-85, 30, order.size)
+        d_loc, d_rem, d_out = dev(vm[:nL]), dev(vm[nL:]), dev(np.full(nL, np.nan))
+        hip.check(L.cdh_fgr_update_local_voltage(h, dp(d_loc), None), "local")
+        hip.check(L.cdh_fgr_update_remote_voltage(h, dp(d_rem), None), "remote")
+        hip.check(L.cdh_fgr_calc(h, dp(d_out), None), "calc")
+        torch.cuda.synchronize()
+        got = d_out.cpu().numpy()
+    finally:
+        L.cdh_fgr_destroy(h)
+    # oracle on the whole anatomy; local cells' stencils only reach x <= 7
+    B = build(t, s6)
+    bx, by, bz = B["shape"]
+    phi = np.zeros((bx, by, bz))
+    c = cells[order]
+    phi[c[:, 0] + 1, c[:, 1] + 1, c[:, 2] + 1] = vm
+    full = apply(B, phi, W.DIFFUSION_SCALE)
+    # map oracle cells (box order) to the local cells
+    bidx = (B["cells"][:, 2] + bz * (B["cells"][:, 1] + by * B["cells"][:, 0]))
+    pos = {int(b): k for k, b in enumerate(bidx)}
+    want = np.array([full[pos[int((cc[2] + 1) + bz * ((cc[1] + 1) + by * (cc[0] + 1)))]]
+                     for cc in c[:nL]])
+    say(f"  bitwise {np.mean(got == want):.4f} max abs {np.abs(got - want).max():.3e}")
+    assert np.array_equal(got, want)
+
+
+def _engine_vs_oracle(anat, celltypes, stims, nsteps, modes=("fp64_unfused", "fp64_fused")):
+    import oracle_lib as OL
+    ref = OL.OracleSim(anat, celltypes, DT, W.DIFFUSION_SCALE, stims)
+    ref.run(nsteps)
+    want_vm = ref.vm()
+    want_act = ref.activation()
+    out = {}
+    for mode in modes:
+        say(f"  engine {mode} ncell={anat.ncell} steps={nsteps}")
+        sim = hip.Simulation(anat, celltypes, DT, W.DIFFUSION_SCALE, stims, mode=mode)
+        sim.run(nsteps)
+        vm = sim.vm()
+        act = sim.activation()
+        assert abs(sim.time - ref.time) == 0.0
+        e = relerr(vm, want_vm)
+        say(f"    Vm max rel {e.max():.3e} bitwise {np.mean(vm == want_vm):.4f}")
+        assert e.max() < 1e-9
+        for k, nm in enumerate(("tact0", "tact", "trec")):
+            same = act[k] == want_act[k]
+            say(f"    {nm} identical {same.mean():.4f}")
+            assert same.all(), nm
+        out[mode] = (vm, sim.state())
+        sim.close()
+    if len(modes) == 2:
+        assert np.array_equal(out[modes[0]][0], out[modes[1]][0])
+        assert np.array_equal(out[modes[0]][1], out[modes[1]][1])
+    ref.close()
+
+
+def test_engine_brick_matches_oracle():
+    say("engine brick 16x12x10 endo, box stimulus x < 4")
+    a = W.brick(16, 12, 10)
+    _engine_vs_oracle(a, ["endoCellML"], [W.box_stimulus(xMax=4, nx=16, ny=12, nz=10)], 1000)
+
+
+def test_engine_shell_three_celltypes_matches_oracle():
+    say("engine ellipsoidal shell 32x28x40, endo/mid/epi, rotating fibres, 8 stimuli")
+    a, depth = W.ellipsoid_shell(32, 28, 40, seed=0)
+    stims = W.endocardial_point_stimuli(a, depth, 8, seed=0)
+    _engine_vs_oracle(a, W.CELLTYPES_EMI, stims, 1000)
+
+
+def test_engine_single_cell_and_ragged_groups():
+    say("engine: 1 cell, and 7-cell groups (gate tail rule)")
+    a = W.brick(1, 1, 1)
+    _engine_vs_oracle(a, ["epiCellML"], [W.box_stimulus(nx=1, ny=1, nz=1)], 300)
+    a = W.brick(3, 3, 3)
+    a.celltype[:] = np.arange(27) % 3
+    _engine_vs_oracle(a, W.CELLTYPES_EMI, [W.box_stimulus(xMax=2, nx=3, ny=3, nz=3)], 300)
