@@ -28,6 +28,7 @@ struct ReactionArgs {
    const double *Vm;
    double *dVm;
    double dt;
+   uint32_t gate_mask;          // all 12 bits set; see gates64
 };
 
 struct StencilArgs {
@@ -124,6 +125,7 @@ struct FusedArgs {
    int off[19];
    double scale;
    double dt;
+   uint32_t gate_mask;          // all 12 bits set; see gates64
    int stim_any;
    const uint8_t *stim_mask;
    StimStep stim;
@@ -150,6 +152,7 @@ __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR
 __global__ void k_fill_f64(int n, double v, double *x);
 __global__ void k_set_list_f64(int n, const int *list, double v, double *x);
 __global__ void k_sensor_f64(SensorArgs a, const double *Vm);
+template <int WAVES>
 __global__ void k_step_fused_f64(FusedArgs a);
 
 }  // namespace cdh

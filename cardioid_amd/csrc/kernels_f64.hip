@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
    for (int s = 0; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
    const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
-   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st);
+   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
    for (int s = 0; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
    a.dVm[e] = dVR;
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
 #pragma unroll
    for (int s = S_M; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
-   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st);
+   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
    for (int s = S_M; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
 }
@@ -120,7 +120,9 @@ __global__ __launch_bounds__(256) void k_sensor_f64(SensorArgs a, const double *
 // One omp-loop step for every tissue cell, Vm in the padded box layout and
 // double buffered: reads box_cur, writes box_next.  Per cell, in order:
 // stimulus value, reaction at Vm_t, 19-point stencil on Vm_t, integrator.
-__global__ __launch_bounds__(256) void k_step_fused_f64(FusedArgs a)
+// WAVES is the occupancy the register allocation targets (waves per SIMD).
+template <int WAVES>
+__global__ __launch_bounds__(256, WAVES) void k_step_fused_f64(FusedArgs a)
 {
    const int i = xcd_cell_index(a.n, a.nblock_per_xcd);
    if (i >= a.n) return;
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void k_step_fused_f64(FusedArgs a)
    for (int s = 0; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
    const ParamSet P = a.params[a.pid[i]];
    const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
-   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st);
+   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
    for (int s = 0; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
 
@@ -149,5 +151,9 @@ __global__ __launch_bounds__(256) void k_step_fused_f64(FusedArgs a)
    a.box_next[b] = Vn;
    if (a.sens.enabled) sensor_eval_pair(a.sens, i, V, Vn);
 }
+
+template __global__ void k_step_fused_f64<2>(FusedArgs);
+template __global__ void k_step_fused_f64<3>(FusedArgs);
+template __global__ void k_step_fused_f64<4>(FusedArgs);
 
 }  // namespace cdh

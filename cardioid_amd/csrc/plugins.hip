@@ -81,6 +81,7 @@ ReactionArgs reaction_args(cdh_tt06 *r, double dt, const int *index, int n, cons
    a.Vm = Vm;
    a.dVm = dVm;
    a.dt = dt;
+   a.gate_mask = 0xFFFu;
    return a;
 }
 

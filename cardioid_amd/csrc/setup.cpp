@@ -27,8 +27,7 @@ std::vector<double> scaled_fit_coefficients(const int *l, const int *m, const do
 {
    CDH_REQUIRE(l && m && coef, "fit set: null argument");
    CDH_REQUIRE(dt > 0, "fit set: dt must be positive");
-   std::vector<double> out;
-   out.reserve(kFitCoefTotal);
+   std::vector<double> out(kFitCoefTotal, 0.0);
    int pos = 0;
    for (int i = 0; i < 33; i++) {
       if (l[i] != kFitOrder[i].l || m[i] != kFitOrder[i].m)
@@ -41,7 +40,7 @@ std::vector<double> scaled_fit_coefficients(const int *l, const int *m, const do
       for (int j = 0; j < l[i] + m[i]; j++) {
          double c = coef[pos + j];
          if (tauR && j < m[i]) c *= dt;
-         out.push_back(c);
+         out[fit_offset(i) + j] = c;
       }
       pos += l[i] + m[i];
    }

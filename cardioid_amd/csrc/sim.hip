@@ -37,6 +37,7 @@ struct cdh_sim {
    int64_t n = 0, stride = 0;
    BoxGeom geom{};
    int mode = CDH_MODE_FP64_FUSED;
+   int variant = 3;                      // fused kernel occupancy target (waves/SIMD)
    hipStream_t stream = nullptr;
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
    TT06Const K{};
@@ -145,6 +146,7 @@ void step_fused(cdh_sim *s)
    for (int k = 0; k < 19; k++) a.off[k] = s->off[k];
    a.scale = s->p.diffusionScale;
    a.dt = s->p.dt;
+   a.gate_mask = 0xFFFu;
    a.stim = stim_step(s, s->time);
    a.stim_any = a.stim.on != 0;
    a.stim_mask = s->stimMask.p;
@@ -153,7 +155,12 @@ void step_fused(cdh_sim *s)
    ++s->loop;
    a.sens = sensor_args(s);
    if (a.n > 0) {
-      hipLaunchKernelGGL(k_step_fused_f64, dim3(8 * a.nblock_per_xcd), dim3(kBlock), 0, s->stream, a);
+      const dim3 grid(8 * a.nblock_per_xcd), block(kBlock);
+      switch (s->variant) {
+      case 2: hipLaunchKernelGGL(k_step_fused_f64<2>, grid, block, 0, s->stream, a); break;
+      case 4: hipLaunchKernelGGL(k_step_fused_f64<4>, grid, block, 0, s->stream, a); break;
+      default: hipLaunchKernelGGL(k_step_fused_f64<3>, grid, block, 0, s->stream, a); break;
+      }
       CDH_LAUNCH_CHECK();
    }
    s->cur ^= 1;
@@ -192,6 +199,7 @@ void step_unfused(cdh_sim *s)
    r.Vm = s->Vm.p;
    r.dVm = s->dVmR.p;
    r.dt = s->p.dt;
+   r.gate_mask = 0xFFFu;
    hipLaunchKernelGGL(k_tt06_calc_f64, g, b, 0, s->stream, r);
    // diffusion: scatter Vm into the box, then the stencil
    hipLaunchKernelGGL(k_scatter_f64, g, b, 0, s->stream, n, s->bi.p, s->Vm.p, s->box[0].p);
@@ -234,6 +242,9 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
       auto s = std::make_unique<cdh_sim>();
       s->p = *p;
       s->mode = p->mode;
+      CDH_REQUIRE(p->variant == 0 || (p->variant >= 2 && p->variant <= 4),
+                  "cdh_sim_create: variant must be 0 (default) or 2..4");
+      s->variant = p->variant ? p->variant : 3;
       const int64_t n = p->ncell;
       s->n = n;
       s->stride = std::max<int64_t>(64, (n + 63) / 64 * 64);

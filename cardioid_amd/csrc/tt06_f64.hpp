@@ -27,8 +27,8 @@ struct ParamSet {
 
 constexpr int kMaxParamSets = 256;
 
-template <int LO, int HI>
-__device__ __forceinline__ double horner64(const double *__restrict__ a, double x)
+template <int LO, int HI, class P>
+__device__ __forceinline__ double horner64(P a, double x)
 {
    double s = a[HI];
 #pragma unroll
@@ -42,11 +42,12 @@ __device__ __forceinline__ double fit64(const double *__restrict__ coef, double 
 {
    constexpr int off = fit_offset(I);
    constexpr int l = kFitOrder[I].l, m = kFitOrder[I].m;
-   const double num = horner64<0, m - 1>(coef + off, x);
+   const double *__restrict__ c = coef + off;
+   const double num = horner64<0, m - 1>(c, x);
    if constexpr (l < 2) {
       return num;
    } else {
-      const double den = horner64<m, m + l - 1>(coef + off, x);
+      const double den = horner64<m, m + l - 1>(c, x);
       return num / den;
    }
 }
@@ -153,24 +154,32 @@ __device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &
 }
 
 // The 12 gate updates at voltage Vm (the voltage of the step's start).
+//
+// `gmask` is a uniform kernel argument with every bit set.  Testing it puts
+// each gate in a basic block of its own, which keeps the compiler from
+// issuing the scalar loads of all 26 gate fits ahead of time (they would
+// not fit in SGPRs and get spilled to VGPR lanes).
 __device__ __forceinline__ void gates64(const double *__restrict__ coef, const double *__restrict__ expc,
-                                        int s_switch, uint8_t flags, double Vm, double *st)
+                                        int s_switch, uint8_t flags, double Vm, double *st,
+                                        uint32_t gmask)
 {
    if (flags & kFlagGatesFixed) return;
    const bool sub = (flags & kFlagFGateSub) != 0;
-   st[S_M] = gate64<0, G_RL>(coef, expc, Vm, st[S_M], false);
-   st[S_H] = gate64<1, G_DIF>(coef, expc, Vm, st[S_H], false);
-   st[S_J] = gate64<2, G_DIF>(coef, expc, Vm, st[S_J], false);
-   st[S_XR1] = gate64<3, G_DIF>(coef, expc, Vm, st[S_XR1], false);
-   st[S_XR2] = gate64<4, G_SUM>(coef, expc, Vm, st[S_XR2], false);
-   st[S_XS] = gate64<5, G_DIF>(coef, expc, Vm, st[S_XS], false);
-   st[S_R] = gate64<6, G_DIF>(coef, expc, Vm, st[S_R], false);
-   st[S_D] = gate64<7, G_DIF>(coef, expc, Vm, st[S_D], false);
-   st[S_F] = gate64<8, G_DIF>(coef, expc, Vm, st[S_F], sub);
-   st[S_F2] = gate64<9, G_DIF>(coef, expc, Vm, st[S_F2], false);
-   st[S_JL] = gate64<10, G_DIF>(coef, expc, Vm, st[S_JL], false);
-   if (s_switch) st[S_S] = gate64<12, G_DIF>(coef, expc, Vm, st[S_S], false);
-   else          st[S_S] = gate64<11, G_DIF>(coef, expc, Vm, st[S_S], false);
+   if (gmask & (1u << 0)) st[S_M] = gate64<0, G_RL>(coef, expc, Vm, st[S_M], false);
+   if (gmask & (1u << 1)) st[S_H] = gate64<1, G_DIF>(coef, expc, Vm, st[S_H], false);
+   if (gmask & (1u << 2)) st[S_J] = gate64<2, G_DIF>(coef, expc, Vm, st[S_J], false);
+   if (gmask & (1u << 3)) st[S_XR1] = gate64<3, G_DIF>(coef, expc, Vm, st[S_XR1], false);
+   if (gmask & (1u << 4)) st[S_XR2] = gate64<4, G_SUM>(coef, expc, Vm, st[S_XR2], false);
+   if (gmask & (1u << 5)) st[S_XS] = gate64<5, G_DIF>(coef, expc, Vm, st[S_XS], false);
+   if (gmask & (1u << 6)) st[S_R] = gate64<6, G_DIF>(coef, expc, Vm, st[S_R], false);
+   if (gmask & (1u << 7)) st[S_D] = gate64<7, G_DIF>(coef, expc, Vm, st[S_D], false);
+   if (gmask & (1u << 8)) st[S_F] = gate64<8, G_DIF>(coef, expc, Vm, st[S_F], sub);
+   if (gmask & (1u << 9)) st[S_F2] = gate64<9, G_DIF>(coef, expc, Vm, st[S_F2], false);
+   if (gmask & (1u << 10)) st[S_JL] = gate64<10, G_DIF>(coef, expc, Vm, st[S_JL], false);
+   if (gmask & (1u << 11)) {
+      if (s_switch) st[S_S] = gate64<12, G_DIF>(coef, expc, Vm, st[S_S], false);
+      else          st[S_S] = gate64<11, G_DIF>(coef, expc, Vm, st[S_S], false);
+   }
 }
 
 }  // namespace cdh

@@ -50,14 +50,19 @@ constexpr FitOrder kFitOrder[33] = {
    {7, 8}, {11, 11},   // s1
 };
 
+// Device layout of the coefficients: fit i starts at fit_offset(i), with one
+// unused slot between consecutive fits.  The gap keeps the compiler from
+// merging the scalar loads of different fits into one wide load issued at
+// kernel entry (which would hold all 468 coefficients in SGPRs at once).
+constexpr int kFitGap = 1;
 constexpr int fit_offset(int i)
 {
    int o = 0;
-   for (int k = 0; k < i; k++) o += kFitOrder[k].l + kFitOrder[k].m;
+   for (int k = 0; k < i; k++) o += kFitOrder[k].l + kFitOrder[k].m + kFitGap;
    return o;
 }
-constexpr int kFitCoefTotal = fit_offset(33);   // 468
-static_assert(kFitCoefTotal == 468, "fit layout");
+constexpr int kFitCoefCount = 468;                // coefficients in a fit file
+constexpr int kFitCoefTotal = fit_offset(33);     // device slots incl. gaps
 
 // Per-cell-type parameters (a reaction object's CellTypeParms).
 struct CellType {

@@ -61,7 +61,7 @@ class SimParams(C.Structure):
                 ("celltype", u8p), ("sigma", dp), ("ncelltype", C.c_int),
                 ("celltypes", C.POINTER(CellType)), ("scales", dp), ("fit_l", ip), ("fit_m", ip),
                 ("fit_coef", dp), ("threshold", C.c_double), ("mode", C.c_int),
-                ("gate_tail_rule", C.c_int), ("sensors", C.c_int)]
+                ("gate_tail_rule", C.c_int), ("sensors", C.c_int), ("variant", C.c_int)]
 
 
 EXPORTS = [
@@ -174,7 +174,7 @@ class Simulation:
     def __init__(self, anatomy, celltypes: Sequence[str], dt: float, diffusion_scale: float,
                  stimuli=(), mode: str = "fp64_fused", threshold: float = -40.0,
                  fits: Optional[FitSet] = None, scales: Optional[np.ndarray] = None,
-                 gate_tail_rule: bool = True, sensors: bool = True):
+                 gate_tail_rule: bool = True, sensors: bool = True, variant: int = 0):
         L = lib()
         self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
         fl, fm, fc = self.fits.packed()
@@ -196,7 +196,8 @@ class Simulation:
                       ncelltype=len(celltypes), celltypes=cts,
                       scales=ptr(sc) if sc is not None else None,
                       fit_l=ptr(fl, ip), fit_m=ptr(fm, ip), fit_coef=ptr(fc), threshold=threshold,
-                      mode=MODES[mode], gate_tail_rule=int(gate_tail_rule), sensors=int(sensors))
+                      mode=MODES[mode], gate_tail_rule=int(gate_tail_rule), sensors=int(sensors),
+                      variant=variant)
         st = (BoxStim * max(1, len(stimuli)))(*stimuli)
         h = C.c_void_p()
         check(L.cdh_sim_create(C.byref(p), len(stimuli), st, C.byref(h)), "cdh_sim_create")

@@ -192,6 +192,7 @@ typedef struct {
    int mode;                    /* CDH_MODE_* */
    int gate_tail_rule;          /* 1: the v2 loops' per-group bounds (reference) */
    int sensors;                 /* 1: record activation times */
+   int variant;                 /* fused kernel occupancy target, waves/SIMD: 2..4; 0 = default */
 } cdh_sim_params;
 
 int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim, cdh_sim **out);
