@@ -31,6 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # centre weight (80)
 FUSED_F64_BYTES = 400
 UNFUSED_F64_BYTES = 480
+# mixed: 17 FP32 states r+w (136), Ca_i/Na_i FP64 r+w (32), Vm r+w (16), 18 FP32 weights (72)
+MIXED_BYTES = 256
 
 
 def log(msg):
@@ -88,6 +90,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--grid", type=str, default="256,256,512")
     ap.add_argument("--unfused-steps", type=int, default=500)
+    ap.add_argument("--mixed-steps", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -105,7 +108,8 @@ def main():
         f"in {time.perf_counter() - t:.1f} s")
 
     results = {}
-    for mode, steps in (("fp64_fused", args.steps), ("fp64_unfused", args.unfused_steps)):
+    for mode, steps in (("fp64_fused", args.steps), ("mixed_fused", args.mixed_steps),
+                        ("fp64_unfused", args.unfused_steps)):
         if steps <= 0:
             continue
         t = time.perf_counter()
@@ -147,6 +151,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
                      "kernel": "k_step_fused_f64", "algorithmic_bytes_per_cell_step": FUSED_F64_BYTES},
     }
+    if "mixed_fused" in results:
+        mx = results["mixed_fused"]
+        mms = mx["ms"] / mx["steps"]
+        out["mixed_fused"] = {"value": mx["rate"], "unit": "Gcell-steps/s", "steps": mx["steps"],
+                              "ms_per_step": mms, "speedup_vs_fp64_fused": mx["rate"] / fused["rate"],
+                              "algorithmic_bytes_per_cell_step": MIXED_BYTES,
+                              "achieved_GBps": MIXED_BYTES * anat.ncell / (mms * 1e-3) / 1e9}
     if "fp64_unfused" in results:
         u = results["fp64_unfused"]
         out["fp64_unfused"] = {"value": u["rate"], "unit": "Gcell-steps/s", "steps": u["steps"],

@@ -25,6 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fopenmp",
 # per-source extra flags
 SOURCES = {
     "kernels_f64.hip": ["-ffp-contract=off"],
+    "kernels_mixed.hip": ["-ffp-contract=fast"],
     "plugins.hip": ["-ffp-contract=off"],
     "sim.hip": ["-ffp-contract=off"],
     "setup.cpp": ["-x", "hip", "-ffp-contract=off"],

@@ -76,10 +76,37 @@ struct SensorArgs {
    double *trec;
    uint8_t *active;             // threshold state (stateful form only)
    uint8_t *activated;          // Vm > 0 seen (stateful form only)
+   // APD90 (no counterpart in the reference): after activation track the
+   // largest Vm; t90 = first time Vm < Vrest + 0.1*(Vmax - Vrest), Vrest =
+   // the cell type's rest Vm
+   int apd;
+   uint8_t *apd_phase;          // 0 waiting, 1 tracking Vmax, 2 done
+   double *vmax;
+   double *t90;
 };
 
+__device__ __forceinline__ void apd_eval(const SensorArgs &a, int i, double v, double vrest,
+                                         bool activated_now)
+{
+   const uint8_t ph = a.apd_phase[i];
+   if (ph == 0) {
+      if (activated_now) {
+         a.apd_phase[i] = 1;
+         a.vmax[i] = v;
+      }
+   } else if (ph == 1) {
+      const double vm = a.vmax[i];
+      if (v > vm) {
+         a.vmax[i] = v;
+      } else if (v < vrest + 0.1 * (vm - vrest)) {
+         a.t90[i] = a.time;
+         a.apd_phase[i] = 2;
+      }
+   }
+}
+
 // Stateful form, the reference's own bookkeeping.
-__device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v)
+__device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v, double vrest)
 {
    if (!a.activated[i] && v > 0) {
       a.activated[i] = 1;
@@ -89,22 +116,33 @@ __device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v
       a.active[i] = 0;
       if (a.trec[i] < 0 && a.tact[i] >= 0) a.trec[i] = a.time;
    }
+   bool now = false;
    if (!a.active[i] && v > a.threshold) {
       a.active[i] = 1;
-      if (a.tact[i] < 0) a.tact[i] = a.time;
+      if (a.tact[i] < 0) {
+         a.tact[i] = a.time;
+         now = true;
+      }
    }
+   if (a.apd) apd_eval(a, i, v, vrest, now);
 }
 
 // Pair form from (Vm before, Vm after) of the step: the state before the
 // step is "Vm above the level" (false before the first evaluation), which
 // equals the stateful form unless Vm lands exactly on the level.
-__device__ __forceinline__ void sensor_eval_pair(const SensorArgs &a, int i, double v0, double v1)
+__device__ __forceinline__ void sensor_eval_pair(const SensorArgs &a, int i, double v0, double v1,
+                                                 double vrest)
 {
    const bool was0 = !a.first_eval && v0 > 0;
    if (!was0 && v1 > 0 && a.tact0[i] < 0) a.tact0[i] = a.time;
    const bool wasT = !a.first_eval && v0 > a.threshold;
-   if (!wasT && v1 > a.threshold && a.tact[i] < 0) a.tact[i] = a.time;
+   bool now = false;
+   if (!wasT && v1 > a.threshold && a.tact[i] < 0) {
+      a.tact[i] = a.time;
+      now = true;
+   }
    if (wasT && v1 < a.threshold && a.trec[i] < 0 && a.tact[i] >= 0) a.trec[i] = a.time;
+   if (a.apd) apd_eval(a, i, v1, vrest, now);
 }
 
 struct FusedArgs {
@@ -132,6 +170,31 @@ struct FusedArgs {
    SensorArgs sens;
 };
 
+struct MixedArgs {
+   TT06Const K;
+   const double *coef;
+   const ParamSet *params;
+   const uint8_t *pid;
+   const uint8_t *flags;
+   float *st32;                 // [17][stride]
+   double *st64;                // [2][stride]: Ca_i, Na_i
+   int64_t stride;
+   int n;
+   int nblock_per_xcd;
+   const uint32_t *bi;
+   const double *box_cur;
+   double *box_next;
+   const float *Aw;
+   int off[19];
+   double scale;
+   double dt;
+   uint32_t gate_mask;
+   int stim_any;
+   const uint8_t *stim_mask;
+   StimStep stim;
+   SensorArgs sens;
+};
+
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
 // XCD owns one contiguous range of cells (and of the box), which keeps the
 // stencil's neighbour rows in that XCD's L2.
@@ -151,8 +214,11 @@ __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR
                                 const double *dVmD, const double *iStim);
 __global__ void k_fill_f64(int n, double v, double *x);
 __global__ void k_set_list_f64(int n, const int *list, double v, double *x);
-__global__ void k_sensor_f64(SensorArgs a, const double *Vm);
+__global__ void k_sensor_f64(SensorArgs a, const double *Vm, const double *vrest);
 template <int WAVES>
 __global__ void k_step_fused_f64(FusedArgs a);
+// kernels (kernels_mixed.hip)
+template <int WAVES>
+__global__ void k_step_fused_mixed(MixedArgs a);
 
 }  // namespace cdh

@@ -99,22 +99,25 @@ __global__ __launch_bounds__(256) void k_integrate_sensor_f64(int n, double dt,
                                                               const double *__restrict__ dVmR,
                                                               const double *__restrict__ dVmD,
                                                               const double *__restrict__ iStim,
-                                                              SensorArgs s)
+                                                              SensorArgs s,
+                                                              const ParamSet *__restrict__ params,
+                                                              const uint8_t *__restrict__ pid)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= n) return;
    const double v0 = Vm[i];
    const double v1 = v0 + dt * (dVmR[i] + dVmD[i] + iStim[i]);
    Vm[i] = v1;
-   if (s.enabled) sensor_eval_pair(s, i, v0, v1);
+   if (s.enabled) sensor_eval_pair(s, i, v0, v1, s.apd ? params[pid[i]].Vm0 : 0.0);
 }
 
 // Sensor evaluation at (time, loop) after the step; Vm in cell order.
-__global__ __launch_bounds__(256) void k_sensor_f64(SensorArgs a, const double *__restrict__ Vm)
+__global__ __launch_bounds__(256) void k_sensor_f64(SensorArgs a, const double *__restrict__ Vm,
+                                                    const double *__restrict__ vrest)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= a.n) return;
-   sensor_eval(a, i, Vm[i]);
+   sensor_eval(a, i, Vm[i], a.apd ? vrest[i] : 0.0);
 }
 
 // One omp-loop step for every tissue cell, Vm in the padded box layout and
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256, WAVES) void k_step_fused_f64(FusedArgs a)
 
    const double Vn = V + a.dt * (dVR + dVD + iStim);
    a.box_next[b] = Vn;
-   if (a.sens.enabled) sensor_eval_pair(a.sens, i, V, Vn);
+   if (a.sens.enabled) sensor_eval_pair(a.sens, i, V, Vn, P.Vm0);
 }
 
 template __global__ void k_step_fused_f64<2>(FusedArgs);

@@ -438,8 +438,12 @@ int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, voi
       a.trec = s->trec.p;
       a.active = s->active.p;
       a.activated = s->activated.p;
+      a.apd = 0;
+      a.apd_phase = nullptr;
+      a.vmax = nullptr;
+      a.t90 = nullptr;
       hipLaunchKernelGGL(k_sensor_f64, dim3(grid_for(s->n, kBlock)), dim3(kBlock), 0, S(stream), a,
-                         Vm);
+                         Vm, (const double *)nullptr);
       CDH_LAUNCH_CHECK();
       s->evaluated = 1;
    });

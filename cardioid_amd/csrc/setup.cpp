@@ -59,6 +59,8 @@ ParamSet make_param_set(const CellType &ct, double s_kr, double s_cal, double s_
    p.g_NaL = ct.g_NaL;
    p.c_CaL = s_cal;
    p.c_Na = s_na * K.c20;
+   p.K0 = ct.state0[S_DVK_I];
+   p.Vm0 = ct.Vm0;
    p.s_switch = ct.s_switch;
    p.pad_ = 0;
    return p;

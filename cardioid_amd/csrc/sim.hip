@@ -26,7 +26,8 @@
 namespace cdh {
 
 __global__ void k_integrate_sensor_f64(int n, double dt, double *Vm, const double *dVmR,
-                                       const double *dVmD, const double *iStim, SensorArgs s);
+                                       const double *dVmD, const double *iStim, SensorArgs s,
+                                       const ParamSet *params, const uint8_t *pid);
 
 }  // namespace cdh
 
@@ -47,7 +48,10 @@ struct cdh_sim {
    DevBuf<double> coef;
    DevBuf<ParamSet> params;
    DevBuf<uint8_t> pid, flags, stimMask;
-   DevBuf<double> state;
+   DevBuf<double> state;              // FP64 modes: [19][stride]
+   DevBuf<float> st32;                // mixed mode: [17][stride]
+   DevBuf<double> st64;               // mixed mode: [2][stride] (Ca_i, Na_i)
+   std::vector<double> K0cell;        // mixed mode: rest K_i per engine cell (host)
    DevBuf<uint32_t> bi;
    DevBuf<double> A0;
    DevBuf<float> Aw;
@@ -59,6 +63,8 @@ struct cdh_sim {
    DevBuf<double> Vm, dVmR, dVmD, iStim;
    // sensors
    DevBuf<double> tact0, tact, trec;
+   DevBuf<uint8_t> apdPhase;
+   DevBuf<double> vmax, t90;
    int off[19] = {0};
    double time = 0.0;
    int loop = 0;
@@ -75,6 +81,70 @@ struct cdh_sim {
 namespace {
 
 constexpr int kBlock = 256;
+
+// mixed-mode FP32 slots (tt06_mixed.hpp order) and the reference slot each holds
+constexpr int mixed_n32 = 17;
+int mixed_slot_source(int q)
+{
+   static const int src[mixed_n32] = {S_CA_SS, S_CA_SR, S_R_PRIME, S_FCASS, -1 /* K_i - K0 */,
+                                      S_M, S_H, S_J, S_XR1, S_XR2, S_XS, S_R, S_D, S_F, S_F2,
+                                      S_JL, S_S};
+   return src[q];
+}
+
+// Vm per engine cell from the current box
+std::vector<double> engine_vm(cdh_sim *s);
+
+// all 19 states per engine cell, [19][stride], the reference's layout
+// (slot 1 = dVK_i = K_i/c9 + Vm)
+std::vector<double> state19(cdh_sim *s)
+{
+   const int64_t n = s->n, S = s->stride;
+   std::vector<double> st((size_t)N_STATE * S, 0.0);
+   if (s->mode != CDH_MODE_MIXED_FUSED) {
+      s->state.download(st.data(), st.size());
+      return st;
+   }
+   std::vector<float> s32((size_t)mixed_n32 * S);
+   std::vector<double> s64((size_t)2 * S);
+   s->st32.download(s32.data(), s32.size());
+   s->st64.download(s64.data(), s64.size());
+   std::vector<double> vm = engine_vm(s);
+   for (int64_t k = 0; k < n; k++) {
+      for (int q = 0; q < mixed_n32; q++) {
+         const int d = mixed_slot_source(q);
+         if (d >= 0) st[d * S + k] = s32[q * S + k];
+      }
+      st[S_CA_I * S + k] = s64[k];
+      st[S_NA_I * S + k] = s64[S + k];
+      const double K_i = s->K0cell[k] + (double)s32[4 * S + k];
+      st[S_DVK_I * S + k] = K_i / s->K.c9 + vm[k];
+   }
+   return st;
+}
+
+void set_state19(cdh_sim *s, const std::vector<double> &st, const std::vector<double> &vm)
+{
+   const int64_t n = s->n, S = s->stride;
+   if (s->mode != CDH_MODE_MIXED_FUSED) {
+      s->state.upload(st.data(), st.size());
+      return;
+   }
+   std::vector<float> s32((size_t)mixed_n32 * S, 0.0f);
+   std::vector<double> s64((size_t)2 * S, 0.0);
+   for (int64_t k = 0; k < n; k++) {
+      for (int q = 0; q < mixed_n32; q++) {
+         const int d = mixed_slot_source(q);
+         if (d >= 0) s32[q * S + k] = (float)st[d * S + k];
+      }
+      const double K_i = s->K.c9 * (st[S_DVK_I * S + k] - vm[k]);
+      s32[4 * S + k] = (float)(K_i - s->K0cell[k]);
+      s64[k] = st[S_CA_I * S + k];
+      s64[S + k] = st[S_NA_I * S + k];
+   }
+   s->st32.upload(s32.data(), s32.size());
+   s->st64.upload(s64.data(), s64.size());
+}
 
 // PeriodicPulse::eval (elec/PeriodicPulse.cc:22-28)
 double pulse(const cdh_box_stim &b, double time)
@@ -109,6 +179,8 @@ void track(cdh_sim *s, DevBuf<T> &b, size_t count)
    s->bytes += (int64_t)(count * sizeof(T));
 }
 
+void step_unfused(cdh_sim *s);
+
 SensorArgs sensor_args(cdh_sim *s)
 {
    SensorArgs a{};
@@ -122,6 +194,10 @@ SensorArgs sensor_args(cdh_sim *s)
    a.trec = s->trec.p;
    a.active = nullptr;
    a.activated = nullptr;
+   a.apd = (s->p.sensors & 2) ? 1 : 0;
+   a.apd_phase = s->apdPhase.p;
+   a.vmax = s->vmax.p;
+   a.t90 = s->t90.p;
    return a;
 }
 
@@ -164,6 +240,54 @@ void step_fused(cdh_sim *s)
       CDH_LAUNCH_CHECK();
    }
    s->cur ^= 1;
+}
+
+void step_mixed(cdh_sim *s)
+{
+   MixedArgs a;
+   a.K = s->K;
+   a.coef = s->coef.p;
+   a.params = s->params.p;
+   a.pid = s->pid.p;
+   a.flags = s->flags.p;
+   a.st32 = s->st32.p;
+   a.st64 = s->st64.p;
+   a.stride = s->stride;
+   a.n = (int)s->n;
+   const int nblock = grid_for(s->n, kBlock);
+   a.nblock_per_xcd = (nblock + 7) / 8;
+   a.bi = s->bi.p;
+   a.box_cur = s->box[s->cur].p;
+   a.box_next = s->box[s->cur ^ 1].p;
+   a.Aw = s->Aw.p;
+   for (int k = 0; k < 19; k++) a.off[k] = s->off[k];
+   a.scale = s->p.diffusionScale;
+   a.dt = s->p.dt;
+   a.gate_mask = 0xFFFu;
+   a.stim = stim_step(s, s->time);
+   a.stim_any = a.stim.on != 0;
+   a.stim_mask = s->stimMask.p;
+   s->time += s->p.dt;
+   ++s->loop;
+   a.sens = sensor_args(s);
+   if (a.n > 0) {
+      const dim3 grid(8 * a.nblock_per_xcd), block(kBlock);
+      switch (s->variant) {
+      case 4: hipLaunchKernelGGL(k_step_fused_mixed<4>, grid, block, 0, s->stream, a); break;
+      case 5: hipLaunchKernelGGL(k_step_fused_mixed<5>, grid, block, 0, s->stream, a); break;
+      case 6: hipLaunchKernelGGL(k_step_fused_mixed<6>, grid, block, 0, s->stream, a); break;
+      default: hipLaunchKernelGGL(k_step_fused_mixed<3>, grid, block, 0, s->stream, a); break;
+      }
+      CDH_LAUNCH_CHECK();
+   }
+   s->cur ^= 1;
+}
+
+void step(cdh_sim *s)
+{
+   if (s->mode == CDH_MODE_FP64_FUSED) step_fused(s);
+   else if (s->mode == CDH_MODE_MIXED_FUSED) step_mixed(s);
+   else step_unfused(s);
 }
 
 void step_unfused(cdh_sim *s)
@@ -218,8 +342,26 @@ void step_unfused(cdh_sim *s)
    s->time += s->p.dt;
    ++s->loop;
    hipLaunchKernelGGL(k_integrate_sensor_f64, g, b, 0, s->stream, n, s->p.dt, s->Vm.p, s->dVmR.p,
-                      s->dVmD.p, s->iStim.p, sensor_args(s));
+                      s->dVmD.p, s->iStim.p, sensor_args(s), s->params.p, s->pid.p);
    CDH_LAUNCH_CHECK();
+}
+
+std::vector<double> engine_vm(cdh_sim *s)
+{
+   const int64_t n = s->n;
+   std::vector<double> v(n);
+   if (n == 0) return v;
+   if (s->mode != CDH_MODE_FP64_UNFUSED) {
+      const BoxGeom &g = s->geom;
+      std::vector<double> box((size_t)g.bx * g.by * g.bz);
+      s->box[s->cur].download(box.data(), box.size());
+      std::vector<uint32_t> bi(n);
+      s->bi.download(bi.data(), n);
+      for (int64_t k = 0; k < n; k++) v[k] = box[bi[k]];
+   } else {
+      s->Vm.download(v.data(), n);
+   }
+   return v;
 }
 
 }  // namespace
@@ -236,15 +378,22 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
                   "cdh_sim_create: 1..8 cell types required");
       CDH_REQUIRE(nstim >= 0 && nstim <= kMaxStim && (nstim == 0 || stim),
                   "cdh_sim_create: at most 8 stimuli");
-      CDH_REQUIRE(p->mode == CDH_MODE_FP64_UNFUSED || p->mode == CDH_MODE_FP64_FUSED,
+      CDH_REQUIRE(p->mode == CDH_MODE_FP64_UNFUSED || p->mode == CDH_MODE_FP64_FUSED ||
+                      p->mode == CDH_MODE_MIXED_FUSED,
                   "cdh_sim_create: unknown mode");
       CDH_REQUIRE(p->dt > 0, "cdh_sim_create: dt must be positive");
       auto s = std::make_unique<cdh_sim>();
       s->p = *p;
       s->mode = p->mode;
-      CDH_REQUIRE(p->variant == 0 || (p->variant >= 2 && p->variant <= 4),
-                  "cdh_sim_create: variant must be 0 (default) or 2..4");
-      s->variant = p->variant ? p->variant : 3;
+      if (p->mode == CDH_MODE_MIXED_FUSED) {
+         CDH_REQUIRE(p->variant == 0 || (p->variant >= 3 && p->variant <= 6),
+                     "cdh_sim_create: mixed variant must be 0 (default) or 3..6");
+         s->variant = p->variant ? p->variant : 3;
+      } else {
+         CDH_REQUIRE(p->variant == 0 || (p->variant >= 2 && p->variant <= 4),
+                     "cdh_sim_create: variant must be 0 (default) or 2..4");
+         s->variant = p->variant ? p->variant : 3;
+      }
       const int64_t n = p->ncell;
       s->n = n;
       s->stride = std::max<int64_t>(64, (n + 63) / 64 * 64);
@@ -338,13 +487,33 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
       s->flags.upload(fl.data(), S);
       track(s.get(), s->bi, S);
       s->bi.upload(bi.data(), S);
-      track(s.get(), s->state, (size_t)N_STATE * S);
-      s->state.upload(st.data(), st.size());
-      track(s.get(), s->A0, S);
-      s->A0.upload(A0.data(), S);
+      if (s->mode == CDH_MODE_MIXED_FUSED) {
+         std::vector<float> s32((size_t)mixed_n32 * S, 0.0f);
+         std::vector<double> s64((size_t)2 * S, 0.0);
+         s->K0cell.assign(S, 0.0);
+         for (int64_t k = 0; k < n; k++) {
+            const CellType &ct = cts[p->celltype[s->perm[k]]];
+            s->K0cell[k] = ct.state0[S_DVK_I];
+            for (int q = 0; q < mixed_n32; q++) {
+               const int src = mixed_slot_source(q);
+               s32[q * S + k] = src < 0 ? 0.0f : (float)ct.state0[src];
+            }
+            s64[0 * S + k] = ct.state0[S_CA_I];
+            s64[1 * S + k] = ct.state0[S_NA_I];
+         }
+         track(s.get(), s->st32, s32.size());
+         s->st32.upload(s32.data(), s32.size());
+         track(s.get(), s->st64, s64.size());
+         s->st64.upload(s64.data(), s64.size());
+      } else {
+         track(s.get(), s->state, (size_t)N_STATE * S);
+         s->state.upload(st.data(), st.size());
+         track(s.get(), s->A0, S);
+         s->A0.upload(A0.data(), S);
+      }
       track(s.get(), s->Aw, 18 * S);
       s->Aw.upload(Aw.data(), Aw.size());
-      const int nbuf = s->mode == CDH_MODE_FP64_FUSED ? 2 : 1;
+      const int nbuf = s->mode == CDH_MODE_FP64_UNFUSED ? 1 : 2;
       for (int k = 0; k < nbuf; k++) {
          track(s.get(), s->box[k], nbox);
          s->box[k].upload(box0.data(), nbox);
@@ -392,6 +561,14 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
       s->tact.upload(neg.data(), S);
       track(s.get(), s->trec, S);
       s->trec.upload(neg.data(), S);
+      if (p->sensors & 2) {
+         track(s.get(), s->apdPhase, S);
+         s->apdPhase.zero();
+         track(s.get(), s->vmax, S);
+         s->vmax.zero();
+         track(s.get(), s->t90, S);
+         s->t90.upload(neg.data(), S);
+      }
       CDH_HIP(hipDeviceSynchronize());
       *out = s.release();
    });
@@ -408,10 +585,7 @@ int cdh_sim_run(cdh_sim *s, int nsteps)
 {
    return guard([&] {
       CDH_REQUIRE(s && nsteps >= 0, "cdh_sim_run: bad arguments");
-      for (int k = 0; k < nsteps; k++) {
-         if (s->mode == CDH_MODE_FP64_FUSED) step_fused(s);
-         else step_unfused(s);
-      }
+      for (int k = 0; k < nsteps; k++) step(s);
    });
 }
 
@@ -428,10 +602,7 @@ int cdh_sim_run_timed(cdh_sim *s, int nsteps, double *ms)
    return guard([&] {
       CDH_REQUIRE(s && ms && nsteps >= 0, "cdh_sim_run_timed: bad arguments");
       CDH_HIP(hipEventRecord(s->ev0, s->stream));
-      for (int k = 0; k < nsteps; k++) {
-         if (s->mode == CDH_MODE_FP64_FUSED) step_fused(s);
-         else step_unfused(s);
-      }
+      for (int k = 0; k < nsteps; k++) step(s);
       CDH_HIP(hipEventRecord(s->ev1, s->stream));
       CDH_HIP(hipEventSynchronize(s->ev1));
       float t = 0.0f;
@@ -445,19 +616,8 @@ int cdh_sim_get_vm(cdh_sim *s, double *host)
    return guard([&] {
       CDH_REQUIRE(s && (host || s->n == 0), "cdh_sim_get_vm: null argument");
       CDH_HIP(hipStreamSynchronize(s->stream));
-      const int64_t n = s->n;
-      std::vector<double> v(n);
-      if (s->mode == CDH_MODE_FP64_FUSED) {
-         const BoxGeom &g = s->geom;
-         std::vector<double> box((size_t)g.bx * g.by * g.bz);
-         s->box[s->cur].download(box.data(), box.size());
-         std::vector<uint32_t> bi(n);
-         s->bi.download(bi.data(), n);
-         for (int64_t k = 0; k < n; k++) v[k] = box[bi[k]];
-      } else {
-         s->Vm.download(v.data(), n);
-      }
-      for (int64_t k = 0; k < n; k++) host[s->perm[k]] = v[k];
+      std::vector<double> v = engine_vm(s);
+      for (int64_t k = 0; k < s->n; k++) host[s->perm[k]] = v[k];
    });
 }
 
@@ -467,8 +627,7 @@ int cdh_sim_get_state(cdh_sim *s, double *host)
       CDH_REQUIRE(s && (host || s->n == 0), "cdh_sim_get_state: null argument");
       CDH_HIP(hipStreamSynchronize(s->stream));
       const int64_t n = s->n, S = s->stride;
-      std::vector<double> st((size_t)N_STATE * S);
-      s->state.download(st.data(), st.size());
+      std::vector<double> st = state19(s);
       for (int q = 0; q < N_STATE; q++)
          for (int64_t k = 0; k < n; k++) host[q * n + s->perm[k]] = st[q * S + k];
    });
@@ -480,24 +639,26 @@ int cdh_sim_set_state(cdh_sim *s, const double *state, const double *Vm)
       CDH_REQUIRE(s, "cdh_sim_set_state: null");
       CDH_HIP(hipStreamSynchronize(s->stream));
       const int64_t n = s->n, S = s->stride;
+      std::vector<double> vm = engine_vm(s);
+      if (Vm)
+         for (int64_t k = 0; k < n; k++) vm[k] = Vm[s->perm[k]];
       if (state) {
-         std::vector<double> st((size_t)N_STATE * S);
-         s->state.download(st.data(), st.size());
+         std::vector<double> st((size_t)N_STATE * S, 0.0);
          for (int q = 0; q < N_STATE; q++)
             for (int64_t k = 0; k < n; k++) st[q * S + k] = state[q * n + s->perm[k]];
-         s->state.upload(st.data(), st.size());
+         set_state19(s, st, vm);
       }
       if (Vm) {
          std::vector<uint32_t> bi(n);
          s->bi.download(bi.data(), n);
-         if (s->mode == CDH_MODE_FP64_FUSED) {
+         if (s->mode != CDH_MODE_FP64_UNFUSED) {
             const BoxGeom &g = s->geom;
             std::vector<double> box((size_t)g.bx * g.by * g.bz, 0.0);
-            for (int64_t k = 0; k < n; k++) box[bi[k]] = Vm[s->perm[k]];
+            for (int64_t k = 0; k < n; k++) box[bi[k]] = vm[k];
             s->box[s->cur].upload(box.data(), box.size());
          } else {
             std::vector<double> v(S, 0.0);
-            for (int64_t k = 0; k < n; k++) v[k] = Vm[s->perm[k]];
+            for (int64_t k = 0; k < n; k++) v[k] = vm[k];
             s->Vm.upload(v.data(), S);
          }
       }
@@ -518,6 +679,18 @@ int cdh_sim_get_activation(cdh_sim *s, double *tact0, double *tact, double *trec
          src[q]->download(v.data(), n);
          for (int64_t k = 0; k < n; k++) dst[q][s->perm[k]] = v[k];
       }
+   });
+}
+
+int cdh_sim_get_t90(cdh_sim *s, double *t90)
+{
+   return guard([&] {
+      CDH_REQUIRE(s && t90, "cdh_sim_get_t90: null argument");
+      CDH_REQUIRE(s->p.sensors & 2, "cdh_sim_get_t90: the engine was created without APD90");
+      CDH_HIP(hipStreamSynchronize(s->stream));
+      std::vector<double> v(s->n);
+      s->t90.download(v.data(), s->n);
+      for (int64_t k = 0; k < s->n; k++) t90[s->perm[k]] = v[k];
    });
 }
 

@@ -21,6 +21,8 @@ namespace cdh {
 // (TT06NonGates.c:405-426), so unit scales give the reference's doubles.
 struct ParamSet {
    double P_NaK, g_Ks, c_Kr, g_to, g_NaL, c_CaL, c_Na;
+   double K0;        // rest K_i of the cell type (the mixed form stores K_i - K0)
+   double Vm0;       // rest Vm of the cell type (APD90's Vrest)
    int s_switch;
    int pad_;
 };

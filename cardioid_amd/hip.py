@@ -19,7 +19,9 @@ LIB_PATH = os.path.join(HERE, "libcardioid_hip.so")
 NSTATE = 19
 MODE_FP64_UNFUSED = 0
 MODE_FP64_FUSED = 1
-MODES = {"fp64_unfused": MODE_FP64_UNFUSED, "fp64_fused": MODE_FP64_FUSED}
+MODE_MIXED_FUSED = 2
+MODES = {"fp64_unfused": MODE_FP64_UNFUSED, "fp64_fused": MODE_FP64_FUSED,
+         "mixed_fused": MODE_MIXED_FUSED}
 
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
@@ -77,6 +79,7 @@ EXPORTS = [
     "cdh_activation_sensor_destroy",
     "cdh_sim_create", "cdh_sim_destroy", "cdh_sim_run", "cdh_sim_sync", "cdh_sim_run_timed",
     "cdh_sim_get_vm", "cdh_sim_get_state", "cdh_sim_set_state", "cdh_sim_get_activation",
+    "cdh_sim_get_t90",
     "cdh_sim_time", "cdh_sim_loop", "cdh_sim_info",
 ]
 
@@ -142,6 +145,7 @@ def lib():
         L.cdh_sim_get_state.argtypes = [C.c_void_p, dp]
         L.cdh_sim_set_state.argtypes = [C.c_void_p, dp, dp]
         L.cdh_sim_get_activation.argtypes = [C.c_void_p, dp, dp, dp]
+        L.cdh_sim_get_t90.argtypes = [C.c_void_p, dp]
         L.cdh_sim_info.argtypes = [C.c_void_p, i64p, ip, ip, ip]
         L.cdh_celltype_standard.argtypes = [C.c_char_p, C.POINTER(CellType)]
         _lib = L
@@ -174,7 +178,7 @@ class Simulation:
     def __init__(self, anatomy, celltypes: Sequence[str], dt: float, diffusion_scale: float,
                  stimuli=(), mode: str = "fp64_fused", threshold: float = -40.0,
                  fits: Optional[FitSet] = None, scales: Optional[np.ndarray] = None,
-                 gate_tail_rule: bool = True, sensors: bool = True, variant: int = 0):
+                 gate_tail_rule: bool = True, sensors: int = 1, variant: int = 0):
         L = lib()
         self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
         fl, fm, fc = self.fits.packed()
@@ -247,6 +251,11 @@ class Simulation:
         check(lib().cdh_sim_get_activation(self.h, ptr(a0), ptr(a), ptr(r)),
               "cdh_sim_get_activation")
         return a0, a, r
+
+    def t90(self) -> np.ndarray:
+        out = np.zeros(self.ncell)
+        check(lib().cdh_sim_get_t90(self.h, ptr(out)), "cdh_sim_get_t90")
+        return out
 
     @property
     def time(self) -> float:

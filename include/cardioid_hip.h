@@ -174,6 +174,8 @@ typedef struct {
 
 #define CDH_MODE_FP64_UNFUSED 0   /* reference-structured: reaction, scatter, stencil, integrate */
 #define CDH_MODE_FP64_FUSED 1     /* one launch per step, same arithmetic */
+#define CDH_MODE_MIXED_FUSED 2    /* one launch per step; FP32 gates/concentrations and stencil,
+                                     FP64 Vm, Ca_i, Na_i */
 
 typedef struct {
    int nx, ny, nz;
@@ -191,8 +193,9 @@ typedef struct {
    double threshold;            /* activation threshold, mV (-40 default) */
    int mode;                    /* CDH_MODE_* */
    int gate_tail_rule;          /* 1: the v2 loops' per-group bounds (reference) */
-   int sensors;                 /* 1: record activation times */
-   int variant;                 /* fused kernel occupancy target, waves/SIMD: 2..4; 0 = default */
+   int sensors;                 /* bit 0: activation times; bit 1: APD90 as well */
+   int variant;                 /* fused kernel occupancy target, waves/SIMD (FP64: 2..4,
+                                   mixed: 3..6); 0 = default */
 } cdh_sim_params;
 
 int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim, cdh_sim **out);
@@ -207,6 +210,11 @@ int cdh_sim_get_vm(cdh_sim *s, double *host);          /* ncell, input order */
 int cdh_sim_get_state(cdh_sim *s, double *host);       /* [19][ncell], input order */
 int cdh_sim_set_state(cdh_sim *s, const double *state, const double *Vm);
 int cdh_sim_get_activation(cdh_sim *s, double *tact0, double *tact, double *trec);
+/* APD90 end time per cell (-1 if not reached); APD90 = t90 - tact.  No
+ * counterpart in the reference: after the first up-crossing of the
+ * threshold, t90 is the first time Vm < Vrest + 0.1*(Vmax - Vrest), with
+ * Vrest the cell type's rest Vm and Vmax the largest Vm since activation. */
+int cdh_sim_get_t90(cdh_sim *s, double *t90);
 double cdh_sim_time(const cdh_sim *s);
 int cdh_sim_loop(const cdh_sim *s);
 /* bytes of device memory held, and the box dimensions used */

@@ -14,8 +14,8 @@
  *   time += dt; ++loop                                (simulationLoop.cc:230-231)
  *   sensors at (time, loop)                           (ActivationTimeSensor.cc:99-111,
  *                                                      ActivationAndRecoverySensor.cc:76-92)
- * APD90 has no counterpart in the reference; its definition is stated at
- * sensors() below.
+ * APD90 has no counterpart in the reference; its definition is stated in
+ * oracle.h (ora_sim_get_t90) and at apd_eval() below.
  */
 #include "oracle.h"
 #include <math.h>
@@ -52,6 +52,8 @@ struct ora_sim {
    double *tact0;
    uint8_t *active;
    double *tactThr, *trecThr;
+   uint8_t *apdPhase;
+   double *vmax, *t90, *vrest;
    double time;
    int loop;
 };
@@ -162,6 +164,14 @@ ora_sim *ora_sim_create(const ora_sim_params *p, int nstim, const ora_box_stim *
    s->tactThr = (double *)malloc(sizeof(double) * (n ? n : 1));
    s->trecThr = (double *)malloc(sizeof(double) * (n ? n : 1));
    for (int64_t c = 0; c < n; c++) s->tact0[c] = s->tactThr[c] = s->trecThr[c] = -1.0;
+   s->apdPhase = (uint8_t *)calloc(n ? n : 1, 1);
+   s->vmax = (double *)calloc(n ? n : 1, sizeof(double));
+   s->t90 = (double *)malloc(sizeof(double) * (n ? n : 1));
+   s->vrest = (double *)malloc(sizeof(double) * (n ? n : 1));
+   for (int64_t c = 0; c < n; c++) {
+      s->t90[c] = -1.0;
+      s->vrest[c] = p->celltypes[p->celltype[c]].Vm0;
+   }
    s->time = 0.0;
    s->loop = 0;
    return s;
@@ -179,6 +189,7 @@ void ora_sim_destroy(ora_sim *s)
    free(s->blockIndex); free(s->A); free(s->A0); free(s->VmBlock);
    free(s->Vm); free(s->dVmR); free(s->dVmD); free(s->iStim); free(s->state);
    free(s->act0); free(s->tact0); free(s->active); free(s->tactThr); free(s->trecThr);
+   free(s->apdPhase); free(s->vmax); free(s->t90); free(s->vrest);
    free(s);
 }
 
@@ -244,6 +255,42 @@ static void reaction(ora_sim *s)
    }
 }
 
+/* APD90 bookkeeping of one cell after an evaluation at `time`.  phase 0:
+ * waiting for activation; 1: activated, tracking Vmax; 2: t90 recorded. */
+static void apd_eval(uint8_t *phase, double *vmax, double *t90, double vrest, double v,
+                     double time, int activated_now)
+{
+   if (*phase == 0) {
+      if (activated_now) { *phase = 1; *vmax = v; }
+   } else if (*phase == 1) {
+      if (v > *vmax) {
+         *vmax = v;
+      } else if (v < vrest + 0.1 * (*vmax - vrest)) {
+         *t90 = time;
+         *phase = 2;
+      }
+   }
+}
+
+void ora_apd_trace(const double *V, int nt, double t0, double dt, double vrest,
+                   double threshold, double *tact, double *t90)
+{
+   uint8_t phase = 0, active = 0;
+   double vmax = 0.0;
+   *tact = -1.0;
+   *t90 = -1.0;
+   for (int k = 1; k < nt; k++) {
+      const double time = t0 + k * dt;
+      int act_now = 0;
+      if (active && V[k] < threshold) active = 0;
+      if (!active && V[k] > threshold) {
+         active = 1;
+         if (*tact < 0) { *tact = time; act_now = 1; }
+      }
+      apd_eval(&phase, &vmax, t90, vrest, V[k], time, act_now);
+   }
+}
+
 static void sensors(ora_sim *s)
 {
    const double thr = s->p.threshold;
@@ -255,10 +302,12 @@ static void sensors(ora_sim *s)
          s->active[c] = 0;
          if (s->trecThr[c] < 0 && s->tactThr[c] >= 0) s->trecThr[c] = s->time;
       }
+      int act_now = 0;
       if (!s->active[c] && v > thr) {
          s->active[c] = 1;
-         if (s->tactThr[c] < 0) s->tactThr[c] = s->time;
+         if (s->tactThr[c] < 0) { s->tactThr[c] = s->time; act_now = 1; }
       }
+      apd_eval(&s->apdPhase[c], &s->vmax[c], &s->t90[c], s->vrest[c], v, s->time, act_now);
    }
 }
 
@@ -297,5 +346,6 @@ void ora_sim_set_state(ora_sim *s, const double *in, const double *Vm)
 void ora_sim_get_activation_vm0(const ora_sim *s, double *out) { memcpy(out, s->tact0, sizeof(double) * s->n); }
 void ora_sim_get_activation_thr(const ora_sim *s, double *out) { memcpy(out, s->tactThr, sizeof(double) * s->n); }
 void ora_sim_get_recovery_thr(const ora_sim *s, double *out) { memcpy(out, s->trecThr, sizeof(double) * s->n); }
+void ora_sim_get_t90(const ora_sim *s, double *out) { memcpy(out, s->t90, sizeof(double) * s->n); }
 double ora_sim_time(const ora_sim *s) { return s->time; }
 int ora_sim_loop(const ora_sim *s) { return s->loop; }

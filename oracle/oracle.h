@@ -153,6 +153,14 @@ void ora_sim_get_activation_vm0(const ora_sim *s, double *out);
 void ora_sim_get_activation_thr(const ora_sim *s, double *out);
 /* first down-crossing after first activation, -1 if never */
 void ora_sim_get_recovery_thr(const ora_sim *s, double *out);
+/* APD90 end: first time after activation (threshold up-crossing) at which
+ * Vm < Vrest + 0.1*(Vmax - Vrest), Vrest = the cell type's rest Vm, Vmax =
+ * the largest Vm since activation; -1 if not reached.  APD90 = t90 - tact. */
+void ora_sim_get_t90(const ora_sim *s, double *out);
+/* the same bookkeeping on one sampled trace V[0..nt), sample k at time
+ * t0 + k*dt (k >= 1 evaluated, as the loop evaluates after each step) */
+void ora_apd_trace(const double *V, int nt, double t0, double dt, double vrest,
+                   double threshold, double *tact, double *t90);
 double ora_sim_time(const ora_sim *s);
 int ora_sim_loop(const ora_sim *s);
 

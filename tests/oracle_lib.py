@@ -103,6 +103,9 @@ def oracle():
                    "ora_sim_get_activation_thr", "ora_sim_get_recovery_thr"):
             getattr(L, fn).argtypes = [C.c_void_p, dp]
         L.ora_sim_set_state.argtypes = [C.c_void_p, dp, dp]
+        L.ora_sim_get_t90.argtypes = [C.c_void_p, dp]
+        L.ora_apd_trace.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    dp, dp]
         L.ora_sim_time.restype = C.c_double
         L.ora_sim_time.argtypes = [C.c_void_p]
         L.ora_sim_loop.argtypes = [C.c_void_p]
@@ -227,6 +230,11 @@ class OracleSim:
         O.ora_sim_get_activation_thr(self.h, ptr(a))
         O.ora_sim_get_recovery_thr(self.h, ptr(r))
         return a0, a, r
+
+    def t90(self):
+        out = np.zeros(self.n)
+        oracle().ora_sim_get_t90(self.h, ptr(out))
+        return out
 
     @property
     def time(self):

@@ -198,3 +198,28 @@ def test_engine_single_cell_and_ragged_groups():
     a = W.brick(3, 3, 3)
     a.celltype[:] = np.arange(27) % 3
     _engine_vs_oracle(a, W.CELLTYPES_EMI, [W.box_stimulus(xMax=2, nx=3, ny=3, nz=3)], 300)
+
+
+def test_engine_apd90_matches_oracle():
+    """APD90 (no reference counterpart, definition pinned in test_oracle_apd)
+    of the FP64 engine equals the oracle's, cell by cell."""
+    import oracle_lib as OL
+    say("engine APD90 vs oracle: shell 16x14x20, 2 stimuli, 18000 steps")
+    a, depth = W.ellipsoid_shell(16, 14, 20, seed=1)
+    stims = W.endocardial_point_stimuli(a, depth, 2, seed=1)
+    n = 18000
+    ref = OL.OracleSim(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims)
+    ref.run(n)
+    want_act, want_t90 = ref.activation()[1], ref.t90()
+    ref.close()
+    sim = hip.Simulation(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims, mode="fp64_fused",
+                         sensors=3)
+    sim.run(n)
+    act, t90 = sim.activation()[1], sim.t90()
+    sim.close()
+    apd = t90 - act
+    say(f"  cells {a.ncell}, repolarised {(t90 > 0).mean():.3f}, APD90 range "
+        f"[{apd[t90 > 0].min():.2f}, {apd[t90 > 0].max():.2f}] ms")
+    assert (t90 > 0).mean() > 0.99
+    assert np.array_equal(act, want_act)
+    assert np.array_equal(t90, want_t90)
