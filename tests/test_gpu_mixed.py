@@ -67,7 +67,9 @@ def test_mixed_state_roundtrip_and_drug_scales():
     a = W.brick(12, 10, 8)
     stim = [W.box_stimulus(xMax=3, nx=12, ny=10, nz=8)]
     rng = np.random.default_rng(2)
-    scales = rng.uniform(0.2, 1.0, (a.ncell, 3))
+    # drug scales come per slab / request: a handful of (s_Kr, s_CaL, s_Na) sets
+    sets = rng.uniform(0.2, 1.0, (6, 3))
+    scales = sets[rng.integers(0, 6, a.ncell)]
     a.celltype[:] = rng.integers(0, 3, a.ncell)
     ref = run(a, stim, "fp64_fused", 6000, scales)
     mix = run(a, stim, "mixed_fused", 6000, scales)
