@@ -206,9 +206,11 @@ __device__ __forceinline__ int xcd_cell_index(int n, int nblock_per_xcd)
 }
 
 // kernels (kernels_f64.hip)
+template <bool BLK>
 __global__ void k_tt06_calc_f64(ReactionArgs a);
 __global__ void k_tt06_gate_f64(ReactionArgs a);
 __global__ void k_scatter_f64(int n, const uint32_t *bi, const double *src, double *box);
+template <bool BLK>
 __global__ void k_fgr_calc_f64(StencilArgs a);
 __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR,
                                 const double *dVmD, const double *iStim);

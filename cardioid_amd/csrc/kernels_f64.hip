@@ -18,6 +18,15 @@
 
 namespace cdh {
 
+// state / weight addressing: BLK = the engine's 64-cell blocks, else plain
+// SoA with a row stride (the drop-in objects)
+template <bool BLK>
+__device__ __forceinline__ int64_t sidx(int q, int64_t i, int nq, int64_t stride)
+{
+   return BLK ? blk_index(q, i, nq) : q * stride + i;
+}
+
+template <bool BLK>
 __global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -26,14 +35,16 @@ __global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
    const double V = a.Vm[e];
    double st[N_STATE];
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
+   for (int s = 0; s < N_STATE; s++) st[s] = a.state[sidx<BLK>(s, i, N_STATE, a.stride)];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
    const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
    gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
+   for (int s = 0; s < N_STATE; s++) a.state[sidx<BLK>(s, i, N_STATE, a.stride)] = st[s];
    a.dVm[e] = dVR;
 }
+template __global__ void k_tt06_calc_f64<false>(ReactionArgs);
+template __global__ void k_tt06_calc_f64<true>(ReactionArgs);
 
 // Gate update only (Reaction::updateGate of the pdr loop): the 12 gates at Vm.
 __global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
@@ -59,6 +70,7 @@ __global__ __launch_bounds__(256) void k_scatter_f64(int n, const uint32_t *__re
    if (i < n) box[bi[i]] = src[i];
 }
 
+template <bool BLK>
 __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,9 +78,11 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
    const double *phi = a.box + a.bi[i];
    double tmp = a.A0[i] * phi[0];
 #pragma unroll
-   for (int k = 1; k < 19; k++) tmp += (double)a.Aw[(k - 1) * a.stride + i] * phi[a.off[k]];
+   for (int k = 1; k < 19; k++) tmp += (double)a.Aw[sidx<BLK>(k - 1, i, 18, a.stride)] * phi[a.off[k]];
    a.dVm[i] = tmp * a.scale;
 }
+template __global__ void k_fgr_calc_f64<false>(StencilArgs);
+template __global__ void k_fgr_calc_f64<true>(StencilArgs);
 
 __global__ __launch_bounds__(256) void k_integrate_f64(int n, double dt, double *__restrict__ Vm,
                                                        const double *__restrict__ dVmR,
@@ -133,18 +147,22 @@ __global__ __launch_bounds__(256, WAVES) void k_step_fused_f64(FusedArgs a)
    const double *phi = a.box_cur + b;
    const double V = phi[0];
 
+   // states and weights stream once per step: 64-cell blocks, non-temporal
+   double *sp = a.state + blk_base(i, N_STATE);
    double st[N_STATE];
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
+   for (int s = 0; s < N_STATE; s++) st[s] = __builtin_nontemporal_load(sp + kBlk * s);
    const ParamSet P = a.params[a.pid[i]];
    const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
    gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
+   for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
 
-   double tmp = a.A0[i] * phi[0];
+   const float *wp = a.Aw + blk_base(i, 18);
+   double tmp = __builtin_nontemporal_load(a.A0 + i) * phi[0];
 #pragma unroll
-   for (int k = 1; k < 19; k++) tmp += (double)a.Aw[(k - 1) * a.stride + i] * phi[a.off[k]];
+   for (int k = 1; k < 19; k++)
+      tmp += (double)__builtin_nontemporal_load(wp + kBlk * (k - 1)) * phi[a.off[k]];
    const double dVD = tmp * a.scale;
 
    double iStim = 0.0;

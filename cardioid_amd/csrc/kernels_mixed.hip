@@ -24,22 +24,27 @@ __global__ __launch_bounds__(256, WAVES) void k_step_fused_mixed(MixedArgs a)
    const double *phi = a.box_cur + b;
    const double V = phi[0];
 
+   // states and weights stream once per step: 64-cell blocks, non-temporal
+   float *p32 = a.st32 + blk_base(i, N_STATE32);
+   double *p64 = a.st64 + blk_base(i, N_STATE64);
    float s32[N_STATE32];
 #pragma unroll
-   for (int s = 0; s < N_STATE32; s++) s32[s] = a.st32[s * a.stride + i];
+   for (int s = 0; s < N_STATE32; s++) s32[s] = __builtin_nontemporal_load(p32 + kBlk * s);
    double s64[N_STATE64];
 #pragma unroll
-   for (int s = 0; s < N_STATE64; s++) s64[s] = a.st64[s * a.stride + i];
+   for (int s = 0; s < N_STATE64; s++) s64[s] = __builtin_nontemporal_load(p64 + kBlk * s);
    const ParamSet P = a.params[a.pid[i]];
    const double K_i = P.K0 + (double)s32[M_DK_I];
    double iK;
    const double dVR = nongateM(a.K, P, a.coef, a.dt, V, K_i, s32, s64, &iK);
    gatesM(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, s32, a.gate_mask);
 
+   const float *wp = a.Aw + blk_base(i, 18);
    float acc = 0.0f;
 #pragma unroll
    for (int k = 1; k < 19; k++)
-      acc = __builtin_fmaf(a.Aw[(k - 1) * a.stride + i], (float)(phi[a.off[k]] - V), acc);
+      acc = __builtin_fmaf(__builtin_nontemporal_load(wp + kBlk * (k - 1)),
+                           (float)(phi[a.off[k]] - V), acc);
    const double dVD = (double)acc * a.scale;
 
    double iStim = 0.0;
@@ -49,9 +54,9 @@ __global__ __launch_bounds__(256, WAVES) void k_step_fused_mixed(MixedArgs a)
    // K_i follows dVK_i = K_i/c9 + Vm of the reference: dK_i = c9*dt*(iK - dVm_D - iStim)
    s32[M_DK_I] = (float)((double)s32[M_DK_I] + (a.dt * a.K.c9) * (iK - dVD - iStim));
 #pragma unroll
-   for (int s = 0; s < N_STATE32; s++) a.st32[s * a.stride + i] = s32[s];
+   for (int s = 0; s < N_STATE32; s++) __builtin_nontemporal_store(s32[s], p32 + kBlk * s);
 #pragma unroll
-   for (int s = 0; s < N_STATE64; s++) a.st64[s * a.stride + i] = s64[s];
+   for (int s = 0; s < N_STATE64; s++) __builtin_nontemporal_store(s64[s], p64 + kBlk * s);
    a.box_next[b] = Vn;
    if (a.sens.enabled) sensor_eval_pair(a.sens, i, V, Vn, P.Vm0);
 }

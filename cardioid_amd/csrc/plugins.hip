@@ -156,7 +156,7 @@ int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const do
       CDH_REQUIRE(n == r->n, "cdh_tt06_calc: n differs from numPoints");
       if (n == 0) return;
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVm, r->flags.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(k_tt06_calc_f64<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -169,7 +169,7 @@ int cdh_tt06_update_nongate(cdh_tt06 *r, double dt, const int *indexArray, int n
       if (n == 0) return;
       // the calc kernel with every gate held fixed is the non-gate update
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVR, r->flagsAllFixed.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(k_tt06_calc_f64<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -353,7 +353,7 @@ int cdh_fgr_calc(cdh_fgr *d, double *dVm, void *stream)
       if (d->nLocal == 0) return;
       StencilArgs a = d->args;
       a.dVm = dVm;
-      hipLaunchKernelGGL(k_fgr_calc_f64, dim3(grid_for(d->nLocal, kBlock)), dim3(kBlock), 0,
+      hipLaunchKernelGGL(k_fgr_calc_f64<false>, dim3(grid_for(d->nLocal, kBlock)), dim3(kBlock), 0,
                          S(stream), a);
       CDH_LAUNCH_CHECK();
    });

@@ -48,9 +48,10 @@ struct cdh_sim {
    DevBuf<double> coef;
    DevBuf<ParamSet> params;
    DevBuf<uint8_t> pid, flags, stimMask;
-   DevBuf<double> state;              // FP64 modes: [19][stride]
-   DevBuf<float> st32;                // mixed mode: [17][stride]
-   DevBuf<double> st64;               // mixed mode: [2][stride] (Ca_i, Na_i)
+   // per-cell multi-field arrays in 64-cell blocks (blk_index), stride cells
+   DevBuf<double> state;              // FP64 modes: 19 fields
+   DevBuf<float> st32;                // mixed mode: 17 fields
+   DevBuf<double> st64;               // mixed mode: 2 fields (Ca_i, Na_i)
    std::vector<double> K0cell;        // mixed mode: rest K_i per engine cell (host)
    DevBuf<uint32_t> bi;
    DevBuf<double> A0;
@@ -95,6 +96,26 @@ int mixed_slot_source(int q)
 // Vm per engine cell from the current box
 std::vector<double> engine_vm(cdh_sim *s);
 
+// Host copies of the engine's multi-field arrays are [nq][S] (S = stride, a
+// multiple of 64); on the device they are in 64-cell blocks (blk_index).
+template <class T>
+void upload_blocked(DevBuf<T> &d, const std::vector<T> &soa, int nq, int64_t S)
+{
+   std::vector<T> b(soa.size());
+   for (int q = 0; q < nq; q++)
+      for (int64_t k = 0; k < S; k++) b[blk_index(q, k, nq)] = soa[q * S + k];
+   d.upload(b.data(), b.size());
+}
+
+template <class T>
+void download_blocked(DevBuf<T> &d, std::vector<T> &soa, int nq, int64_t S)
+{
+   std::vector<T> b(soa.size());
+   d.download(b.data(), b.size());
+   for (int q = 0; q < nq; q++)
+      for (int64_t k = 0; k < S; k++) soa[q * S + k] = b[blk_index(q, k, nq)];
+}
+
 // all 19 states per engine cell, [19][stride], the reference's layout
 // (slot 1 = dVK_i = K_i/c9 + Vm)
 std::vector<double> state19(cdh_sim *s)
@@ -102,13 +123,13 @@ std::vector<double> state19(cdh_sim *s)
    const int64_t n = s->n, S = s->stride;
    std::vector<double> st((size_t)N_STATE * S, 0.0);
    if (s->mode != CDH_MODE_MIXED_FUSED) {
-      s->state.download(st.data(), st.size());
+      download_blocked(s->state, st, N_STATE, S);
       return st;
    }
    std::vector<float> s32((size_t)mixed_n32 * S);
    std::vector<double> s64((size_t)2 * S);
-   s->st32.download(s32.data(), s32.size());
-   s->st64.download(s64.data(), s64.size());
+   download_blocked(s->st32, s32, mixed_n32, S);
+   download_blocked(s->st64, s64, 2, S);
    std::vector<double> vm = engine_vm(s);
    for (int64_t k = 0; k < n; k++) {
       for (int q = 0; q < mixed_n32; q++) {
@@ -127,7 +148,7 @@ void set_state19(cdh_sim *s, const std::vector<double> &st, const std::vector<do
 {
    const int64_t n = s->n, S = s->stride;
    if (s->mode != CDH_MODE_MIXED_FUSED) {
-      s->state.upload(st.data(), st.size());
+      upload_blocked(s->state, st, N_STATE, S);
       return;
    }
    std::vector<float> s32((size_t)mixed_n32 * S, 0.0f);
@@ -142,8 +163,8 @@ void set_state19(cdh_sim *s, const std::vector<double> &st, const std::vector<do
       s64[k] = st[S_CA_I * S + k];
       s64[S + k] = st[S_NA_I * S + k];
    }
-   s->st32.upload(s32.data(), s32.size());
-   s->st64.upload(s64.data(), s64.size());
+   upload_blocked(s->st32, s32, mixed_n32, S);
+   upload_blocked(s->st64, s64, 2, S);
 }
 
 // PeriodicPulse::eval (elec/PeriodicPulse.cc:22-28)
@@ -324,7 +345,7 @@ void step_unfused(cdh_sim *s)
    r.dVm = s->dVmR.p;
    r.dt = s->p.dt;
    r.gate_mask = 0xFFFu;
-   hipLaunchKernelGGL(k_tt06_calc_f64, g, b, 0, s->stream, r);
+   hipLaunchKernelGGL(k_tt06_calc_f64<true>, g, b, 0, s->stream, r);
    // diffusion: scatter Vm into the box, then the stencil
    hipLaunchKernelGGL(k_scatter_f64, g, b, 0, s->stream, n, s->bi.p, s->Vm.p, s->box[0].p);
    StencilArgs d;
@@ -337,7 +358,7 @@ void step_unfused(cdh_sim *s)
    for (int k = 0; k < 19; k++) d.off[k] = s->off[k];
    d.scale = s->p.diffusionScale;
    d.dVm = s->dVmD.p;
-   hipLaunchKernelGGL(k_fgr_calc_f64, g, b, 0, s->stream, d);
+   hipLaunchKernelGGL(k_fgr_calc_f64<true>, g, b, 0, s->stream, d);
    // integrator and sensors
    s->time += s->p.dt;
    ++s->loop;
@@ -502,17 +523,17 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
             s64[1 * S + k] = ct.state0[S_NA_I];
          }
          track(s.get(), s->st32, s32.size());
-         s->st32.upload(s32.data(), s32.size());
+         upload_blocked(s->st32, s32, mixed_n32, S);
          track(s.get(), s->st64, s64.size());
-         s->st64.upload(s64.data(), s64.size());
+         upload_blocked(s->st64, s64, 2, S);
       } else {
          track(s.get(), s->state, (size_t)N_STATE * S);
-         s->state.upload(st.data(), st.size());
+         upload_blocked(s->state, st, N_STATE, S);
          track(s.get(), s->A0, S);
          s->A0.upload(A0.data(), S);
       }
       track(s.get(), s->Aw, 18 * S);
-      s->Aw.upload(Aw.data(), Aw.size());
+      upload_blocked(s->Aw, Aw, 18, S);
       const int nbuf = s->mode == CDH_MODE_FP64_UNFUSED ? 1 : 2;
       for (int k = 0; k < nbuf; k++) {
          track(s.get(), s->box[k], nbox);

@@ -13,6 +13,8 @@
 //   cell types            elec/TT06Func.cc:421-612
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace cdh {
@@ -63,6 +65,20 @@ constexpr int fit_offset(int i)
 }
 constexpr int kFitCoefCount = 468;                // coefficients in a fit file
 constexpr int kFitCoefTotal = fit_offset(33);     // device slots incl. gaps
+
+// Engine layout of per-cell arrays with several fields (states, weights):
+// blocks of 64 cells (one wavefront), field-major inside a block, so that a
+// wavefront's fields are one contiguous run of memory.  Field q of cell i of
+// an array with nq fields sits at blk_base(i, nq) + 64*q.
+constexpr int kBlk = 64;
+__host__ __device__ inline int64_t blk_base(int64_t i, int nq)
+{
+   return (i >> 6) * (int64_t)nq * kBlk + (i & (kBlk - 1));
+}
+__host__ __device__ inline int64_t blk_index(int64_t q, int64_t i, int nq)
+{
+   return blk_base(i, nq) + (int64_t)kBlk * q;
+}
 
 // Per-cell-type parameters (a reaction object's CellTypeParms).
 struct CellType {
