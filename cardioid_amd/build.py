@@ -25,9 +25,13 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fopenmp",
 # per-source extra flags
 SOURCES = {
     "kernels_f64.hip": ["-ffp-contract=off"],
-    "kernels_mixed.hip": ["-ffp-contract=fast"],
+    "engine_f64.hip": ["-ffp-contract=off"],
+    "engine_mixed.hip": ["-ffp-contract=fast"],
+    "engine_host.hip": ["-ffp-contract=off"],
+    "slab.cpp": ["-x", "hip", "-ffp-contract=off"],
     "plugins.hip": ["-ffp-contract=off"],
     "sim.hip": ["-ffp-contract=off"],
+    "batch.hip": ["-ffp-contract=off"],
     "setup.cpp": ["-x", "hip", "-ffp-contract=off"],
     "fgr_weights.cpp": ["-x", "hip", "-ffp-contract=off"],
     "tt06_host.cpp": ["-x", "hip", "-ffp-contract=off"],

@@ -1,0 +1,207 @@
+// engine.hpp -- the fused step over many tissue slabs (device side).
+//
+// The engine advances any number of independent slabs (anatomies with their
+// own box, stimuli, drug scales and clock) in one launch per step.  Cells of
+// all slabs live in one arena of 64-cell tiles; each tile belongs to one
+// slab (blk_slab), so everything per slab is wave-uniform.  Nothing that
+// changes from step to step is a kernel argument: a slab's time and loop
+// count are double buffered in device memory by step parity, which is what
+// lets a run of steps be captured once in a HIP graph and replayed.
+//
+// One step for one cell is the omp-loop step of elec/simulationLoop.cc:
+// 170-241: stimulus (BoxStimulus.cc:53-64, PeriodicPulse.cc:22-28),
+// reaction at Vm_t (TT06Dev_Reaction.cc:393-407), FGR stencil on Vm_t
+// (FGRDiffusionOMP.cc:109-131), Vm += dt*(dVmR + dVmD + iStim)
+// (simulationLoop.cc:221-227), time += dt (:230), then the activation
+// sensors at the new time (ActivationTimeSensor.cc:99-111,
+// ActivationAndRecoverySensor.cc:76-92).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+#include "tt06_f64.hpp"
+#include "tt06_model.hpp"
+
+namespace cdh {
+
+constexpr int kSlabStim = 8;                 // box stimuli per slab (mask bits)
+constexpr uint16_t kNoSlab = 0xFFFF;         // tile not in use
+// stencil byte offsets are stored shifted by this many box points so that
+// they are non-negative (the buffer instructions' SGPR offset is unsigned)
+constexpr int64_t kBoxShift = int64_t(1) << 22;
+
+// per-cell metadata word
+constexpr uint32_t kMetaPad = 0x8000u;       // padding cell of a slab's last tile
+__host__ __device__ inline uint32_t meta_pack(uint32_t pid, uint32_t flags, uint32_t stim)
+{
+   return (pid & 0xFFu) | ((flags & 0xFFu) << 8) | ((stim & 0xFFu) << 16);
+}
+
+// PeriodicPulse with its stimulus window (elec/PeriodicPulse.cc:22-28,
+// elec/Stimulus.hh:21-27)
+struct PulseDev {
+   double t0, tf, period, phase, duration, amplitude;
+};
+
+// One slab.  A step of parity p reads time[p], loop[p] and the stimulus
+// values of that time (stim_on[p], stim_val[p]); the slab's first tile
+// writes time[p^1] = time[p] + dt, loop[p^1] = loop[p] + 1 and the stimulus
+// values at time[p^1].
+struct alignas(16) SlabDev {
+   double time[2];
+   int32_t loop[2];
+   int32_t active;              // 0: the slab's tiles are skipped
+   int32_t first_blk;           // tile that advances the clock
+   uint32_t soff[19];           // (offset_k + kBoxShift) * 8, NodeLocation order
+   int32_t param_base;          // parameter sets of this slab start here
+   int32_t nstim;
+   uint32_t stim_on[2];         // bit k: stimulus k is nonzero at time[p]
+   int32_t pad_[2];
+   double scale;                // diffusionScale
+   double threshold;            // activation threshold
+   double stim_val[2][kSlabStim];
+   PulseDev stim[kSlabStim];
+};
+
+struct EngineArgs {
+   // constants in device memory rather than kernel arguments: the kernel
+   // would otherwise load all of them into SGPRs at entry and spill them
+   const TT06Const *K;
+   const double *coef;
+   const ParamSet *params;
+   const uint32_t *meta;        // per cell: meta_pack(pid, flags, stim mask)
+   const uint32_t *bi;          // per cell: index in the arena's box
+   const uint16_t *blk_slab;    // per 64-cell tile: slab, or kNoSlab
+   SlabDev *slabs;
+   double *state;               // FP64: 19 fields in 64-cell blocks
+   const double *A0;            // FP64: centre weight per cell
+   float *st32;                 // mixed: 17 fields in 64-cell blocks
+   double *st64;                // mixed: Ca_i, Na_i in 64-cell blocks
+   const float *Aw;             // 18 off-centre weights in 64-cell blocks
+   const double *box_cur;       // Vm in the slabs' padded boxes (read)
+   double *box_next;            // (written)
+   int parity;
+   int ntile;                   // 64-cell tiles in use
+   int groups_per_xcd;          // workgroup tiles per XCD
+   double dt;
+   uint32_t gate_mask;          // all 12 bits set; see gates64
+   int sensors;                 // bit 0: activation times, bit 1: APD90
+   double *tact0, *tact, *trec;
+   uint8_t *apd_phase;
+   double *vmax, *t90;
+};
+
+// PeriodicPulse::eval at `time` inside the window t0 < time < tf, else 0
+// (host and device round alike: no contraction, IEEE division).
+__host__ __device__ inline double pulse_value(const PulseDev &b, double time)
+{
+#pragma clang fp contract(off)
+   if (!(time > b.t0 && time < b.tf)) return 0.0;
+   const double t = time - floor(time / b.period) * b.period;
+   if (t >= b.phase && t < b.phase + b.duration) return b.amplitude;
+   return 0.0;
+}
+
+// Stimulus values of every stimulus of a slab at `time`.
+__host__ __device__ inline void slab_stim_eval(const PulseDev *stim, int nstim, double time,
+                                               uint32_t *on, double *val)
+{
+   *on = 0;
+   for (int k = 0; k < kSlabStim; k++) {
+      val[k] = 0.0;
+      if (k < nstim) {
+         val[k] = pulse_value(stim[k], time);
+         if (val[k] != 0.0) *on |= 1u << k;
+      }
+   }
+}
+
+// Wave-uniform values of the slab a tile belongs to.
+struct SlabView {
+   double time1;                // time after the step
+   int first_eval;
+   uint32_t stim_on;
+   double scale, threshold;
+   int param_base;
+};
+
+__device__ __forceinline__ void slab_view(const SlabDev &S, int parity, double dt, SlabView &v)
+{
+#pragma clang fp contract(off)
+   v.time1 = S.time[parity] + dt;
+   v.first_eval = S.loop[parity] == 0;
+   v.scale = S.scale;
+   v.threshold = S.threshold;
+   v.param_base = S.param_base;
+   v.stim_on = S.stim_on[parity];
+}
+
+// The cell's stimulus; later stimuli overwrite earlier ones, as BoxStimulus
+// assigns (elec/BoxStimulus.cc:61-62), so the highest set bit wins.
+__device__ __forceinline__ double slab_stim(const SlabDev &S, const SlabView &v, int parity,
+                                            uint32_t meta)
+{
+   const uint32_t on = (meta >> 16) & v.stim_on & 0xFFu;
+   if (!on) return 0.0;
+   return S.stim_val[parity][31 - __builtin_clz(on)];
+}
+
+// The first tile's lane 0 advances the slab's clock after the step.
+__device__ __forceinline__ void slab_advance(SlabDev &S, int parity, const SlabView &v)
+{
+   const int q = parity ^ 1;
+   S.time[q] = v.time1;
+   S.loop[q] = S.loop[parity] + 1;
+   uint32_t on;
+   double val[kSlabStim];
+   slab_stim_eval(S.stim, S.nstim, v.time1, &on, val);
+   for (int k = 0; k < kSlabStim; k++) S.stim_val[q][k] = val[k];
+   S.stim_on[q] = on;
+}
+
+// Sensors at the time after the step, pair form (Vm before and after).
+__device__ __forceinline__ void engine_sensors(const EngineArgs &a, const SlabView &v, int64_t i,
+                                               double v0, double v1, double vrest)
+{
+   SensorArgs s;
+   s.n = 0;
+   s.enabled = a.sensors;
+   s.first_eval = v.first_eval;
+   s.time = v.time1;
+   s.threshold = v.threshold;
+   s.tact0 = a.tact0;
+   s.tact = a.tact;
+   s.trec = a.trec;
+   s.active = nullptr;
+   s.activated = nullptr;
+   s.apd = (a.sensors >> 1) & 1;
+   s.apd_phase = a.apd_phase;
+   s.vmax = a.vmax;
+   s.t90 = a.t90;
+   sensor_eval_pair(s, (int)i, v0, v1, vrest);
+}
+
+// Buffer resource over a box so that (box index)*8 is the VGPR offset and
+// a shifted stencil offset is the SGPR offset.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(const double *box)
+{
+   // raw buffer, no stride, the whole 32-bit range (the host keeps arena
+   // boxes below 2^28 points); third dword as for every gfx9 target
+   return __builtin_amdgcn_make_buffer_rsrc((void *)(box - kBoxShift), (short)0, -1, 0x00020000);
+}
+
+__device__ __forceinline__ double box_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// kernels (engine_f64.hip, engine_mixed.hip); WAVES = target waves per SIMD
+template <int WAVES>
+__global__ void k_engine_f64(EngineArgs a);
+template <int WAVES>
+__global__ void k_engine_mixed(EngineArgs a);
+
+}  // namespace cdh

@@ -1,0 +1,84 @@
+// engine_f64.hip -- the fused FP64 step over the engine's slabs (gfx950).
+//
+// Compiled with -ffp-contract=off: the reaction, the stencil and the Vm
+// update round as the reference CPU path does, operation by operation
+// (TT06Dev_Reaction::calc, elec/TT06Dev_Reaction.cc:393-407;
+// FGRDiffusionOMP::calc, elec/FGRDiffusionOMP.cc:109-131; the integrator,
+// elec/simulationLoop.cc:221-227).
+#pragma clang fp contract(off)
+
+#include "engine.hpp"
+
+namespace cdh {
+
+namespace {
+
+__device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, const SlabView &v,
+                                         int blk, int lane)
+{
+   const int64_t i = (int64_t)blk * kBlk + lane;
+   // padding cells of a slab's last tile compute on a copy of a tissue
+   // cell's box point and store only their own states, so the tile runs
+   // without a divergent exit
+   const uint32_t meta = a.meta[i];
+   const bool live = !(meta & kMetaPad);
+   const uint32_t b = a.bi[i];
+   const uint32_t b8 = b * 8u;
+   const __amdgpu_buffer_rsrc_t R = box_rsrc(a.box_cur);
+   const double V = box_load(R, b8, S.soff[0]);
+
+   // states and weights stream once per step (non-temporal); the tile's
+   // base address is wave-uniform
+   double *sp = a.state + (int64_t)blk * (N_STATE * kBlk) + lane;
+   double st[N_STATE];
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) st[s] = __builtin_nontemporal_load(sp + kBlk * s);
+   const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
+   const double dVR = nongate64(*a.K, P, a.coef, a.dt, V, st);
+   gates64(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, st, a.gate_mask);
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
+
+   const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
+   double tmp = __builtin_nontemporal_load(a.A0 + i) * V;
+#pragma unroll
+   for (int k = 1; k < 19; k++)
+      tmp += (double)__builtin_nontemporal_load(wp + kBlk * (k - 1)) * box_load(R, b8, S.soff[k]);
+   const double dVD = tmp * v.scale;
+   const double iStim = slab_stim(S, v, a.parity, meta);
+   const double Vn = V + a.dt * (dVR + dVD + iStim);
+   if (live) {
+      a.box_next[b] = Vn;
+      if (a.sensors) engine_sensors(a, v, i, V, Vn, P.Vm0);
+   }
+}
+
+}  // namespace
+
+// Workgroups are dealt round-robin to the 8 XCDs; workgroup w serves a tile
+// group in the contiguous range of XCD (w mod 8), so that each XCD's L2
+// holds the box rows its cells read.  The grid is 8 * groups_per_xcd.
+template <int WAVES>
+__global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
+{
+   const int xcd = blockIdx.x & 7;
+   const int wpg = blockDim.x >> 6;
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int blk = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
+   if (blk >= a.ntile) return;
+   const int sl = a.blk_slab[blk];
+   if (sl == kNoSlab) return;
+   SlabDev &S = a.slabs[sl];
+   if (!S.active) return;
+   SlabView v;
+   slab_view(S, a.parity, a.dt, v);
+   cell_f64(a, S, v, blk, lane);
+   if (blk == S.first_blk && lane == 0) slab_advance(S, a.parity, v);
+}
+
+template __global__ void k_engine_f64<2>(EngineArgs);
+template __global__ void k_engine_f64<3>(EngineArgs);
+template __global__ void k_engine_f64<4>(EngineArgs);
+
+}  // namespace cdh

@@ -1,0 +1,672 @@
+// engine_host.hip -- the multi-slab fused engine (host side); see
+// engine_host.hpp.  Compiled with -ffp-contract=off so that the host copy of
+// each slab's clock rounds as the device copy does.
+#include "engine_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+#include "setup.hpp"
+
+namespace cdh {
+
+// ---------------------------------------------------------------- RangeAlloc
+
+RangeAlloc::RangeAlloc(int64_t capacity) : cap_(capacity)
+{
+   if (capacity > 0) free_[0] = capacity;
+}
+
+int64_t RangeAlloc::alloc(int64_t len)
+{
+   if (len <= 0) return -1;
+   for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < len) continue;
+      const int64_t off = it->first, rest = it->second - len;
+      free_.erase(it);
+      if (rest > 0) free_[off + len] = rest;
+      taken_[off] = len;
+      used_ += len;
+      return off;
+   }
+   return -1;
+}
+
+void RangeAlloc::release(int64_t off, int64_t len)
+{
+   auto t = taken_.find(off);
+   if (t == taken_.end() || t->second != len) throw Error(kErrState, "RangeAlloc: bad release");
+   taken_.erase(t);
+   used_ -= len;
+   auto next = free_.lower_bound(off);
+   if (next != free_.end() && next->first == off + len) {
+      len += next->second;
+      free_.erase(next);
+   }
+   auto it = free_.lower_bound(off);
+   if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == off) {
+         prev->second += len;
+         return;
+      }
+   }
+   free_[off] = len;
+}
+
+int64_t RangeAlloc::end() const
+{
+   if (taken_.empty()) return 0;
+   const auto &last = *taken_.rbegin();
+   return last.first + last.second;
+}
+
+// ---------------------------------------------------------------- helpers
+
+int mixed_slot_source(int q)
+{
+   static const int src[kMixed32] = {S_CA_SS, S_CA_SR, S_R_PRIME, S_FCASS, -1,
+                                     S_M, S_H, S_J, S_XR1, S_XR2, S_XS, S_R, S_D, S_F, S_F2,
+                                     S_JL, S_S};
+   return src[q];
+}
+
+namespace {
+
+using KernelFn = void (*)(EngineArgs);
+
+KernelFn kernel_for(int mode, int waves)
+{
+   if (mode == CDH_MODE_MIXED_FUSED) {
+      switch (waves) {
+      case 4: return k_engine_mixed<4>;
+      case 5: return k_engine_mixed<5>;
+      default: return k_engine_mixed<3>;
+      }
+   }
+   switch (waves) {
+   case 2: return k_engine_f64<2>;
+   case 4: return k_engine_f64<4>;
+   default: return k_engine_f64<3>;
+   }
+}
+
+// a slab's [nq][n] array (slab order) from its blocked device range
+template <class T>
+std::vector<T> unblock(const std::vector<T> &b, int nq, int64_t n)
+{
+   std::vector<T> out((size_t)nq * n);
+   for (int q = 0; q < nq; q++)
+      for (int64_t k = 0; k < n; k++) out[q * n + k] = b[blk_index(q, k, nq)];
+   return out;
+}
+
+template <class T>
+std::vector<T> block(const std::vector<T> &soa, int nq, int64_t S, int64_t stride)
+{
+   std::vector<T> b((size_t)nq * S);
+   for (int q = 0; q < nq; q++)
+      for (int64_t k = 0; k < S; k++) b[blk_index(q, k, nq)] = soa[q * stride + k];
+   return b;
+}
+
+template <class T>
+void download(const T *dev, T *host, size_t count)
+{
+   if (count) CDH_HIP(hipMemcpy(host, dev, count * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+}  // namespace
+
+std::vector<LaunchCfg> Engine::candidates(int mode)
+{
+   std::vector<LaunchCfg> c;
+   const int w0 = mode == CDH_MODE_MIXED_FUSED ? 3 : 2;
+   for (int w = w0; w <= w0 + 2; w++)
+      for (int b : {64, 128, 256}) c.push_back(LaunchCfg{w, b});
+   return c;
+}
+
+// ---------------------------------------------------------------- Engine
+
+template <class T>
+void Engine::track(DevBuf<T> &b, size_t count)
+{
+   b.alloc(count);
+   bytes_ += (int64_t)(count * sizeof(T));
+}
+
+template <class T>
+void Engine::put(T *dst, const std::vector<T> &src)
+{
+   if (src.empty()) return;
+   auto keep = std::make_shared<std::vector<T>>(src);
+   CDH_HIP(hipMemcpyAsync(dst, keep->data(), keep->size() * sizeof(T), hipMemcpyHostToDevice,
+                          stream_));
+   staging_.push_back(keep);
+   CDH_HIP(hipEventRecord(staging_ev_, stream_));
+}
+
+void Engine::retire_staging()
+{
+   if (staging_.empty()) return;
+   if (hipEventQuery(staging_ev_) == hipSuccess) staging_.clear();
+}
+
+Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const double *fit_coef,
+               int64_t cap_cells, int64_t cap_box, int cap_slabs, int cap_params, int sensors)
+    : mode_(mode), dt_(dt), sensors_(sensors)
+{
+   CDH_REQUIRE(mode == CDH_MODE_FP64_FUSED || mode == CDH_MODE_MIXED_FUSED,
+               "engine: mode must be FP64 fused or mixed fused");
+   CDH_REQUIRE(dt > 0, "engine: dt must be positive");
+   CDH_REQUIRE(fit_l && fit_m && fit_coef, "engine: null fit arrays");
+   CDH_REQUIRE(cap_cells >= 1 && cap_cells < (int64_t(1) << 31) - 64,
+               "engine: cell capacity must be 1..2^31-65");
+   CDH_REQUIRE(cap_box >= 1 && cap_box < (int64_t(1) << 28), "engine: box capacity must be below 2^28");
+   CDH_REQUIRE(cap_slabs >= 1 && cap_slabs < kNoSlab, "engine: slab capacity must be 1..65534");
+   CDH_REQUIRE(cap_params >= 1, "engine: parameter-set capacity must be positive");
+   cfg_.waves = mode == CDH_MODE_MIXED_FUSED ? 3 : 3;
+   cfg_.block = 256;
+   const int64_t ntile = (cap_cells + kBlk - 1) / kBlk;
+   const int64_t S = ntile * kBlk;
+   tiles_ = RangeAlloc(ntile);
+   boxes_ = RangeAlloc(cap_box);
+   params_ = RangeAlloc(cap_params);
+   slabs_.resize(cap_slabs);
+
+   CDH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+   CDH_HIP(hipEventCreate(&ev0_));
+   CDH_HIP(hipEventCreate(&ev1_));
+   CDH_HIP(hipEventCreateWithFlags(&staging_ev_, hipEventDisableTiming));
+
+   tt06_constants(&K_);
+   track(dK_, 1);
+   dK_.upload(&K_, 1);
+   std::vector<double> coef = scaled_fit_coefficients(fit_l, fit_m, fit_coef, dt);
+   track(coef_, coef.size());
+   coef_.upload(coef.data(), coef.size());
+   track(psets_, cap_params);
+   track(meta_, S);
+   track(bi_, S);
+   track(blk_slab_, ntile);
+   CDH_HIP(hipMemset(blk_slab_.p, 0xFF, ntile * sizeof(uint16_t)));
+   track(slabdev_, cap_slabs);
+   slabdev_.zero();
+   if (mode == CDH_MODE_FP64_FUSED) {
+      track(state_, (size_t)N_STATE * S);
+      track(A0_, S);
+   } else {
+      track(st32_, (size_t)kMixed32 * S);
+      track(st64_, (size_t)2 * S);
+   }
+   track(Aw_, (size_t)18 * S);
+   for (int k = 0; k < 2; k++) {
+      track(box_[k], cap_box);
+      box_[k].zero();
+   }
+   if (sensors & 1) {
+      track(tact0_, S);
+      track(tact_, S);
+      track(trec_, S);
+   }
+   if (sensors & 2) {
+      track(apd_phase_, S);
+      track(vmax_, S);
+      track(t90_, S);
+   }
+   CDH_HIP(hipDeviceSynchronize());
+}
+
+Engine::~Engine()
+{
+   if (stream_) (void)hipStreamSynchronize(stream_);
+   if (graph_) (void)hipGraphExecDestroy(graph_);
+   if (ev0_) (void)hipEventDestroy(ev0_);
+   if (ev1_) (void)hipEventDestroy(ev1_);
+   if (staging_ev_) (void)hipEventDestroy(staging_ev_);
+   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+bool Engine::fits(int64_t ntile, int64_t nbox, int64_t nparam) const
+{
+   auto room = [](const RangeAlloc &a, int64_t len) {
+      RangeAlloc c = a;
+      return c.alloc(len) >= 0;
+   };
+   bool slot = false;
+   for (const Slab &s : slabs_) slot |= !s.live;
+   return slot && room(tiles_, ntile) && room(boxes_, nbox) && room(params_, nparam);
+}
+
+int Engine::nslab() const
+{
+   int k = 0;
+   for (const Slab &s : slabs_) k += s.live;
+   return k;
+}
+
+void Engine::check_slab(int id) const
+{
+   CDH_REQUIRE(id >= 0 && id < (int)slabs_.size() && slabs_[id].live, "engine: no such slab");
+}
+
+const Engine::Slab &Engine::slab(int id) const
+{
+   check_slab(id);
+   return slabs_[id];
+}
+
+int Engine::add(const SlabHost &h)
+{
+   retire_staging();
+   int id = -1;
+   for (int k = 0; k < (int)slabs_.size(); k++)
+      if (!slabs_[k].live) {
+         id = k;
+         break;
+      }
+   if (id < 0) return -1;
+   const int64_t t0 = tiles_.alloc(h.ntile);
+   const int64_t b0 = t0 < 0 ? -1 : boxes_.alloc(h.nbox);
+   const int64_t p0 = b0 < 0 ? -1 : params_.alloc((int64_t)h.psets.size());
+   if (t0 < 0 || b0 < 0 || p0 < 0) {
+      if (b0 >= 0) boxes_.release(b0, h.nbox);
+      if (t0 >= 0) tiles_.release(t0, h.ntile);
+      return -1;
+   }
+   const int64_t S = h.S, c0 = t0 * kBlk;
+   Slab &s = slabs_[id];
+   s = Slab{};
+   s.live = true;
+   s.tile0 = t0;
+   s.ntile = h.ntile;
+   s.box0 = b0;
+   s.nbox = h.nbox;
+   s.param0 = p0;
+   s.nparam = (int64_t)h.psets.size();
+   s.n = h.n;
+   s.geom = h.geom;
+   s.perm = h.perm;
+   s.bi.assign(h.bi.begin(), h.bi.begin() + h.n);
+   s.K0.resize(h.n);
+   for (int64_t k = 0; k < h.n; k++) s.K0[k] = h.psets[h.meta[k] & 0xFFu].K0;
+   s.sensors = sensors_;
+
+   put(psets_.p + p0, h.psets);
+   put(meta_.p + c0, h.meta);
+   std::vector<uint32_t> bi(S);
+   for (int64_t k = 0; k < S; k++) bi[k] = (uint32_t)(b0 + h.bi[k]);
+   put(bi_.p + c0, bi);
+   if (mode_ == CDH_MODE_FP64_FUSED) {
+      put(state_.p + c0 * N_STATE, block(h.state, N_STATE, S, S));
+      put(A0_.p + c0, h.A0);
+   } else {
+      std::vector<float> s32((size_t)kMixed32 * S);
+      std::vector<double> s64((size_t)2 * S);
+      for (int64_t k = 0; k < S; k++) {
+         for (int q = 0; q < kMixed32; q++) {
+            const int src = mixed_slot_source(q);
+            s32[q * S + k] = src < 0 ? 0.0f : (float)h.state[src * S + k];
+         }
+         s64[k] = h.state[S_CA_I * S + k];
+         s64[S + k] = h.state[S_NA_I * S + k];
+      }
+      put(st32_.p + c0 * kMixed32, block(s32, kMixed32, S, S));
+      put(st64_.p + c0 * 2, block(s64, 2, S, S));
+   }
+   put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
+   put(box_[0].p + b0, h.box0);
+   put(box_[1].p + b0, h.box0);
+   if (sensors_ & 1) {
+      const std::vector<double> neg(S, -1.0);
+      put(tact0_.p + c0, neg);
+      put(tact_.p + c0, neg);
+      put(trec_.p + c0, neg);
+   }
+   if (sensors_ & 2) {
+      put(apd_phase_.p + c0, std::vector<uint8_t>(S, 0));
+      put(vmax_.p + c0, std::vector<double>(S, 0.0));
+      put(t90_.p + c0, std::vector<double>(S, -1.0));
+   }
+   SlabDev d{};
+   d.time[parity_] = 0.0;
+   d.loop[parity_] = 0;
+   d.active = 1;
+   d.first_blk = (int32_t)t0;
+   for (int k = 0; k < 19; k++) d.soff[k] = (uint32_t)((h.off[k] + kBoxShift) * 8);
+   d.param_base = (int32_t)p0;
+   d.nstim = (int32_t)h.stim.size();
+   d.scale = h.diffusionScale;
+   d.threshold = h.threshold;
+   for (int q = 0; q < d.nstim; q++) {
+      const cdh_box_stim &b = h.stim[q];
+      d.stim[q] = PulseDev{b.t0, b.tf, b.period, b.phase, b.duration, b.amplitude};
+   }
+   slab_stim_eval(d.stim, d.nstim, 0.0, &d.stim_on[parity_], d.stim_val[parity_]);
+   put(slabdev_.p + id, std::vector<SlabDev>{d});
+   put(blk_slab_.p + t0, std::vector<uint16_t>(h.ntile, (uint16_t)id));
+   return id;
+}
+
+void Engine::remove(int id)
+{
+   check_slab(id);
+   retire_staging();
+   Slab &s = slabs_[id];
+   put(blk_slab_.p + s.tile0, std::vector<uint16_t>(s.ntile, kNoSlab));
+   put(&slabdev_.p[id].active, std::vector<int32_t>{0});
+   tiles_.release(s.tile0, s.ntile);
+   boxes_.release(s.box0, s.nbox);
+   params_.release(s.param0, s.nparam);
+   s = Slab{};
+}
+
+EngineArgs Engine::args(int parity) const
+{
+   EngineArgs a{};
+   a.K = dK_.p;
+   a.coef = coef_.p;
+   a.params = psets_.p;
+   a.meta = meta_.p;
+   a.bi = bi_.p;
+   a.blk_slab = blk_slab_.p;
+   a.slabs = slabdev_.p;
+   a.state = state_.p;
+   a.A0 = A0_.p;
+   a.st32 = st32_.p;
+   a.st64 = st64_.p;
+   a.Aw = Aw_.p;
+   a.box_cur = box_[parity].p;
+   a.box_next = box_[parity ^ 1].p;
+   a.parity = parity;
+   a.ntile = (int)tiles_.end();
+   const int wpg = cfg_.block / kBlk;
+   const int ngroup = (a.ntile + wpg - 1) / wpg;
+   a.groups_per_xcd = (ngroup + 7) / 8;
+   a.dt = dt_;
+   a.gate_mask = 0xFFFu;
+   a.sensors = sensors_;
+   a.tact0 = tact0_.p;
+   a.tact = tact_.p;
+   a.trec = trec_.p;
+   a.apd_phase = apd_phase_.p;
+   a.vmax = vmax_.p;
+   a.t90 = t90_.p;
+   return a;
+}
+
+void Engine::launch_step(int parity)
+{
+   const EngineArgs a = args(parity);
+   if (a.ntile == 0) return;
+   const dim3 grid(8 * a.groups_per_xcd), blk(cfg_.block);
+   hipLaunchKernelGGL(kernel_for(mode_, cfg_.waves), grid, blk, 0, stream_, a);
+   CDH_LAUNCH_CHECK();
+}
+
+void Engine::step_eager()
+{
+   launch_step(parity_);
+   parity_ ^= 1;
+   for (Slab &s : slabs_)
+      if (s.live) {
+         s.time = s.time + dt_;
+         ++s.loop;
+      }
+}
+
+void Engine::drop_graph()
+{
+   if (graph_) (void)hipGraphExecDestroy(graph_);
+   graph_ = nullptr;
+   graph_ntile_ = -1;
+}
+
+void Engine::rebuild_graph()
+{
+   drop_graph();
+   hipGraph_t g = nullptr;
+   CDH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+   try {
+      for (int k = 0; k < graph_steps_; k++) launch_step(k & 1);
+   } catch (...) {
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+   }
+   CDH_HIP(hipStreamEndCapture(stream_, &g));
+   hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+   (void)hipGraphDestroy(g);
+   CDH_HIP(e);
+   graph_ntile_ = (int)tiles_.end();
+   graph_cfg_ = cfg_;
+}
+
+void Engine::set_graph_steps(int k)
+{
+   CDH_REQUIRE(k >= 0 && k % 2 == 0, "engine: graph steps must be even (0 disables graphs)");
+   graph_steps_ = k;
+   drop_graph();
+}
+
+void Engine::set_launch(const LaunchCfg &c)
+{
+   const int w0 = mode_ == CDH_MODE_MIXED_FUSED ? 3 : 2;
+   CDH_REQUIRE(c.waves >= w0 && c.waves <= w0 + 2, "engine: no kernel for that occupancy");
+   CDH_REQUIRE(c.block == 64 || c.block == 128 || c.block == 256,
+               "engine: workgroup size must be 64, 128 or 256");
+   cfg_ = c;
+   drop_graph();
+}
+
+void Engine::run(int nsteps)
+{
+   CDH_REQUIRE(nsteps >= 0, "engine: negative step count");
+   retire_staging();
+   while (nsteps > 0) {
+      if (graph_steps_ > 0 && parity_ == 0 && nsteps >= graph_steps_) {
+         if (!graph_ || graph_ntile_ != (int)tiles_.end() || graph_cfg_.waves != cfg_.waves ||
+             graph_cfg_.block != cfg_.block)
+            rebuild_graph();
+         CDH_HIP(hipGraphLaunch(graph_, stream_));
+         for (Slab &s : slabs_)
+            if (s.live)
+               for (int k = 0; k < graph_steps_; k++) {
+                  s.time = s.time + dt_;
+                  ++s.loop;
+               }
+         nsteps -= graph_steps_;
+      } else {
+         step_eager();
+         --nsteps;
+      }
+   }
+}
+
+double Engine::run_timed(int nsteps)
+{
+   CDH_HIP(hipEventRecord(ev0_, stream_));
+   run(nsteps);
+   CDH_HIP(hipEventRecord(ev1_, stream_));
+   CDH_HIP(hipEventSynchronize(ev1_));
+   float ms = 0.0f;
+   CDH_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+   return ms;
+}
+
+void Engine::sync()
+{
+   CDH_HIP(hipStreamSynchronize(stream_));
+   retire_staging();
+}
+
+double Engine::autotune(int steps, std::vector<std::pair<LaunchCfg, double>> *table)
+{
+   CDH_REQUIRE(steps >= 1, "engine: autotune needs at least one step per candidate");
+   sync();
+   // snapshot everything a step writes, restore it after each candidate
+   const int64_t S = tiles_.end() * kBlk, nb = boxes_.end();
+   std::vector<std::pair<void *, size_t>> regions;
+   auto add = [&](void *p, size_t bytes) {
+      if (p && bytes) regions.emplace_back(p, bytes);
+   };
+   add(state_.p, sizeof(double) * N_STATE * S);
+   add(st32_.p, sizeof(float) * kMixed32 * S);
+   add(st64_.p, sizeof(double) * 2 * S);
+   add(box_[0].p, sizeof(double) * nb);
+   add(box_[1].p, sizeof(double) * nb);
+   add(tact0_.p, sizeof(double) * S);
+   add(tact_.p, sizeof(double) * S);
+   add(trec_.p, sizeof(double) * S);
+   add(apd_phase_.p, S);
+   add(vmax_.p, sizeof(double) * S);
+   add(t90_.p, sizeof(double) * S);
+   add(slabdev_.p, sizeof(SlabDev) * slabs_.size());
+   std::vector<std::unique_ptr<DevBuf<uint8_t>>> saved;
+   for (auto &r : regions) {
+      saved.emplace_back(new DevBuf<uint8_t>(r.second));
+      CDH_HIP(hipMemcpyAsync(saved.back()->p, r.first, r.second, hipMemcpyDeviceToDevice, stream_));
+   }
+   const int par0 = parity_;
+   std::vector<Slab> host0 = slabs_;
+   const LaunchCfg keep = cfg_;
+   const int gsteps = graph_steps_;
+   graph_steps_ = 0;
+   double best = -1.0;
+   LaunchCfg bestc = keep;
+   for (const LaunchCfg &c : candidates(mode_)) {
+      set_launch(c);
+      run(1);
+      const double ms = run_timed(steps) / steps;
+      if (table) table->emplace_back(c, ms);
+      if (best < 0 || ms < best) {
+         best = ms;
+         bestc = c;
+      }
+      for (size_t k = 0; k < regions.size(); k++)
+         CDH_HIP(hipMemcpyAsync(regions[k].first, saved[k]->p, regions[k].second,
+                                hipMemcpyDeviceToDevice, stream_));
+      parity_ = par0;
+      slabs_ = host0;
+   }
+   CDH_HIP(hipStreamSynchronize(stream_));
+   set_launch(bestc);
+   graph_steps_ = gsteps;
+   return best;
+}
+
+// ---------------------------------------------------------------- accessors
+
+std::vector<double> Engine::vm(int id)
+{
+   check_slab(id);
+   sync();
+   const Slab &s = slabs_[id];
+   std::vector<double> box(s.nbox);
+   download(box_[parity_].p + s.box0, box.data(), s.nbox);
+   std::vector<double> v(s.n);
+   for (int64_t k = 0; k < s.n; k++) v[k] = box[s.bi[k]];
+   return v;
+}
+
+void Engine::set_vm(int id, const std::vector<double> &v)
+{
+   check_slab(id);
+   sync();
+   const Slab &s = slabs_[id];
+   std::vector<double> box(s.nbox);
+   download(box_[parity_].p + s.box0, box.data(), s.nbox);
+   for (int64_t k = 0; k < s.n; k++) box[s.bi[k]] = v[k];
+   put(box_[parity_].p + s.box0, box);
+   sync();
+}
+
+std::vector<double> Engine::state19(int id)
+{
+   check_slab(id);
+   sync();
+   const Slab &s = slabs_[id];
+   const int64_t S = s.ntile * kBlk, c0 = s.tile0 * kBlk, n = s.n;
+   if (mode_ == CDH_MODE_FP64_FUSED) {
+      std::vector<double> b((size_t)N_STATE * S);
+      download(state_.p + c0 * N_STATE, b.data(), b.size());
+      return unblock(b, N_STATE, n);
+   }
+   std::vector<float> b32((size_t)kMixed32 * S);
+   std::vector<double> b64((size_t)2 * S);
+   download(st32_.p + c0 * kMixed32, b32.data(), b32.size());
+   download(st64_.p + c0 * 2, b64.data(), b64.size());
+   const std::vector<float> s32 = unblock(b32, kMixed32, n);
+   const std::vector<double> s64 = unblock(b64, 2, n);
+   const std::vector<double> v = vm(id);
+   std::vector<double> st((size_t)N_STATE * n, 0.0);
+   for (int64_t k = 0; k < n; k++) {
+      for (int q = 0; q < kMixed32; q++) {
+         const int d = mixed_slot_source(q);
+         if (d >= 0) st[d * n + k] = s32[q * n + k];
+      }
+      st[S_CA_I * n + k] = s64[k];
+      st[S_NA_I * n + k] = s64[n + k];
+      const double K_i = s.K0[k] + (double)s32[4 * n + k];
+      st[S_DVK_I * n + k] = K_i / K_.c9 + v[k];
+   }
+   return st;
+}
+
+void Engine::set_state19(int id, const std::vector<double> &st, const std::vector<double> &v)
+{
+   check_slab(id);
+   sync();
+   const Slab &s = slabs_[id];
+   const int64_t S = s.ntile * kBlk, c0 = s.tile0 * kBlk, n = s.n;
+   // padding cells keep their states: start from the device copy
+   if (mode_ == CDH_MODE_FP64_FUSED) {
+      std::vector<double> b((size_t)N_STATE * S);
+      download(state_.p + c0 * N_STATE, b.data(), b.size());
+      for (int q = 0; q < N_STATE; q++)
+         for (int64_t k = 0; k < n; k++) b[blk_index(q, k, N_STATE)] = st[q * n + k];
+      put(state_.p + c0 * N_STATE, b);
+   } else {
+      std::vector<float> b32((size_t)kMixed32 * S);
+      std::vector<double> b64((size_t)2 * S);
+      download(st32_.p + c0 * kMixed32, b32.data(), b32.size());
+      download(st64_.p + c0 * 2, b64.data(), b64.size());
+      for (int64_t k = 0; k < n; k++) {
+         for (int q = 0; q < kMixed32; q++) {
+            const int d = mixed_slot_source(q);
+            if (d >= 0) b32[blk_index(q, k, kMixed32)] = (float)st[d * n + k];
+         }
+         const double K_i = K_.c9 * (st[S_DVK_I * n + k] - v[k]);
+         b32[blk_index(4, k, kMixed32)] = (float)(K_i - s.K0[k]);
+         b64[blk_index(0, k, 2)] = st[S_CA_I * n + k];
+         b64[blk_index(1, k, 2)] = st[S_NA_I * n + k];
+      }
+      put(st32_.p + c0 * kMixed32, b32);
+      put(st64_.p + c0 * 2, b64);
+   }
+   sync();
+}
+
+void Engine::activation(int id, double *tact0, double *tact, double *trec)
+{
+   check_slab(id);
+   CDH_REQUIRE(sensors_ & 1, "engine: created without activation sensors");
+   sync();
+   const Slab &s = slabs_[id];
+   const int64_t c0 = s.tile0 * kBlk;
+   if (tact0) download(tact0_.p + c0, tact0, s.n);
+   if (tact) download(tact_.p + c0, tact, s.n);
+   if (trec) download(trec_.p + c0, trec, s.n);
+}
+
+void Engine::t90(int id, double *out)
+{
+   check_slab(id);
+   CDH_REQUIRE(sensors_ & 2, "engine: created without APD90");
+   sync();
+   const Slab &s = slabs_[id];
+   download(t90_.p + s.tile0 * kBlk, out, s.n);
+}
+
+}  // namespace cdh

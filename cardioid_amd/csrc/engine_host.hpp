@@ -1,0 +1,148 @@
+// engine_host.hpp -- the multi-slab fused engine (host side).
+//
+// Owns one arena of device memory sized at creation: 64-cell tiles, box
+// points, slab records and parameter sets.  Slabs are added and removed
+// while others run (first-fit ranges; every transfer is ordered on the
+// engine's stream), all active slabs advance together, one launch per step,
+// and a run of steps can be replayed from a captured HIP graph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "engine.hpp"
+#include "hip_util.hpp"
+#include "slab.hpp"
+
+namespace cdh {
+
+// first-fit allocator of [0, capacity) ranges
+class RangeAlloc {
+ public:
+   explicit RangeAlloc(int64_t capacity = 0);
+   int64_t alloc(int64_t len);            // -1 if no range fits
+   void release(int64_t off, int64_t len);
+   int64_t end() const;                   // one past the highest allocated unit
+   int64_t used() const { return used_; }
+   int64_t capacity() const { return cap_; }
+
+ private:
+   std::map<int64_t, int64_t> free_;      // offset -> length
+   std::map<int64_t, int64_t> taken_;
+   int64_t cap_ = 0, used_ = 0;
+};
+
+struct LaunchCfg {
+   int waves = 0;     // occupancy target of the kernel instance (waves per SIMD)
+   int block = 256;   // threads per workgroup: 64, 128 or 256
+};
+
+class Engine {
+ public:
+   struct Slab {
+      bool live = false;
+      int64_t tile0 = 0, ntile = 0, box0 = 0, nbox = 0, param0 = 0, nparam = 0;
+      int64_t n = 0;
+      BoxGeom geom{};
+      std::vector<int64_t> perm;
+      std::vector<uint32_t> bi;        // n: box index within the slab's box
+      std::vector<double> K0;          // n: rest K_i (mixed state conversion)
+      int sensors = 0;
+      double time = 0.0;               // host copy of the slab clock
+      int loop = 0;
+   };
+
+   // mode: CDH_MODE_FP64_FUSED or CDH_MODE_MIXED_FUSED
+   Engine(int mode, double dt, const int *fit_l, const int *fit_m, const double *fit_coef,
+          int64_t cap_cells, int64_t cap_box, int cap_slabs, int cap_params, int sensors);
+   ~Engine();
+   Engine(const Engine &) = delete;
+   Engine &operator=(const Engine &) = delete;
+
+   // returns the slab id, or -1 if the arena has no room for it
+   int add(const SlabHost &h);
+   void remove(int id);
+   bool fits(int64_t ntile, int64_t nbox, int64_t nparam) const;
+
+   void run(int nsteps);
+   double run_timed(int nsteps);          // ms between events around the steps
+   void sync();
+   void set_launch(const LaunchCfg &c);
+   LaunchCfg launch() const { return cfg_; }
+   void set_graph_steps(int k);           // 0: launch each step; else replay k-step graphs
+   // times each launch configuration over `steps` steps (after a warm-up
+   // step) from the same starting state, keeps the fastest; returns its ms
+   double autotune(int steps, std::vector<std::pair<LaunchCfg, double>> *table);
+
+   const Slab &slab(int id) const;
+   std::vector<double> vm(int id);                        // slab order
+   std::vector<double> state19(int id);                   // [19][n], slab order
+   void set_state19(int id, const std::vector<double> &st, const std::vector<double> &vm);
+   void set_vm(int id, const std::vector<double> &vm);
+   void activation(int id, double *tact0, double *tact, double *trec);   // slab order
+   void t90(int id, double *out);
+
+   int mode() const { return mode_; }
+   int64_t device_bytes() const { return bytes_; }
+   int64_t cells_used() const { return tiles_.used() * kBlk; }
+   int64_t box_used() const { return boxes_.used(); }
+   int nslab() const;
+   hipStream_t stream() const { return stream_; }
+   static std::vector<LaunchCfg> candidates(int mode);
+
+ private:
+   void step_eager();
+   void launch_step(int parity);
+   void rebuild_graph();
+   void drop_graph();
+   EngineArgs args(int parity) const;
+   void retire_staging();
+   template <class T>
+   void put(T *dst, const std::vector<T> &src);           // async H2D via kept staging
+   template <class T>
+   void track(DevBuf<T> &b, size_t count);
+   void check_slab(int id) const;
+   std::vector<double> st_download_blocked(double *dev, int nq, const Slab &s);
+
+   int mode_;
+   double dt_;
+   int sensors_;
+   hipStream_t stream_ = nullptr;
+   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+   TT06Const K_{};
+   LaunchCfg cfg_;
+   int parity_ = 0;
+   int graph_steps_ = 0;
+   hipGraphExec_t graph_ = nullptr;
+   int graph_ntile_ = -1;
+   LaunchCfg graph_cfg_;
+   int64_t bytes_ = 0;
+
+   RangeAlloc tiles_, boxes_, params_;
+   std::vector<Slab> slabs_;
+   std::vector<std::shared_ptr<void>> staging_;
+   hipEvent_t staging_ev_ = nullptr;
+
+   DevBuf<TT06Const> dK_;
+   DevBuf<double> coef_;
+   DevBuf<ParamSet> psets_;
+   DevBuf<uint32_t> meta_, bi_;
+   DevBuf<uint16_t> blk_slab_;
+   DevBuf<SlabDev> slabdev_;
+   DevBuf<double> state_, A0_, st64_;
+   DevBuf<float> st32_, Aw_;
+   DevBuf<double> box_[2];
+   DevBuf<double> tact0_, tact_, trec_, vmax_, t90_;
+   DevBuf<uint8_t> apd_phase_;
+};
+
+// mixed-mode FP32 slots (tt06_mixed.hpp order) and the reference slot each
+// holds; -1 is K_i - K0
+int mixed_slot_source(int q);
+constexpr int kMixed32 = 17;
+
+}  // namespace cdh

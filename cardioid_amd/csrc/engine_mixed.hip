@@ -1,0 +1,97 @@
+// engine_mixed.hip -- the mixed-precision fused step over the engine's slabs
+// (gfx950).
+//
+// Same step as engine_f64.hip with the storage and arithmetic of
+// tt06_mixed.hpp and an FP32 stencil on voltage differences,
+//    dVm_D = scale * sum_{k=1..18} A_k * (phi_k - phi_0),
+// which is the FGR operator itself (its centre weight is minus the sum of
+// the others, elec/FGRDiffusionOMP.cc:214-219) without the centre weight.
+#pragma clang fp contract(fast)
+
+#include "engine.hpp"
+#include "tt06_mixed.hpp"
+
+namespace cdh {
+
+using namespace mixed;
+
+namespace {
+
+__device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S, const SlabView &v,
+                                           int blk, int lane)
+{
+   const int64_t i = (int64_t)blk * kBlk + lane;
+   // padding cells of a slab's last tile compute on a copy of a tissue
+   // cell's box point and store only their own states, so the tile runs
+   // without a divergent exit
+   const uint32_t meta = a.meta[i];
+   const bool live = !(meta & kMetaPad);
+   const uint32_t b = a.bi[i];
+   const uint32_t b8 = b * 8u;
+   const __amdgpu_buffer_rsrc_t R = box_rsrc(a.box_cur);
+   const double V = box_load(R, b8, S.soff[0]);
+
+   // states and weights stream once per step (non-temporal); the tile's
+   // base addresses are wave-uniform
+   float *p32 = a.st32 + (int64_t)blk * (N_STATE32 * kBlk) + lane;
+   double *p64 = a.st64 + (int64_t)blk * (N_STATE64 * kBlk) + lane;
+   float s32[N_STATE32];
+#pragma unroll
+   for (int s = 0; s < N_STATE32; s++) s32[s] = __builtin_nontemporal_load(p32 + kBlk * s);
+   double s64[N_STATE64];
+#pragma unroll
+   for (int s = 0; s < N_STATE64; s++) s64[s] = __builtin_nontemporal_load(p64 + kBlk * s);
+   const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
+   const double K_i = P.K0 + (double)s32[M_DK_I];
+   double iK;
+   const double dVR = nongateM(*a.K, P, a.coef, a.dt, V, K_i, s32, s64, &iK);
+   gatesM(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, s32, a.gate_mask);
+
+   const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
+   float acc = 0.0f;
+#pragma unroll
+   for (int k = 1; k < 19; k++)
+      acc = __builtin_fmaf(__builtin_nontemporal_load(wp + kBlk * (k - 1)),
+                           (float)(box_load(R, b8, S.soff[k]) - V), acc);
+   const double dVD = (double)acc * v.scale;
+   const double iStim = slab_stim(S, v, a.parity, meta);
+   const double Vn = V + a.dt * (dVR + dVD + iStim);
+   // K_i follows dVK_i = K_i/c9 + Vm of the reference: dK_i = c9*dt*(iK - dVm_D - iStim)
+   s32[M_DK_I] = (float)((double)s32[M_DK_I] + (a.dt * a.K->c9) * (iK - dVD - iStim));
+#pragma unroll
+   for (int s = 0; s < N_STATE32; s++) __builtin_nontemporal_store(s32[s], p32 + kBlk * s);
+#pragma unroll
+   for (int s = 0; s < N_STATE64; s++) __builtin_nontemporal_store(s64[s], p64 + kBlk * s);
+   if (live) {
+      a.box_next[b] = Vn;
+      if (a.sensors) engine_sensors(a, v, i, V, Vn, P.Vm0);
+   }
+}
+
+}  // namespace
+
+// Tile-to-XCD mapping as k_engine_f64.
+template <int WAVES>
+__global__ __launch_bounds__(256, WAVES) void k_engine_mixed(EngineArgs a)
+{
+   const int xcd = blockIdx.x & 7;
+   const int wpg = blockDim.x >> 6;
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int blk = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
+   if (blk >= a.ntile) return;
+   const int sl = a.blk_slab[blk];
+   if (sl == kNoSlab) return;
+   SlabDev &S = a.slabs[sl];
+   if (!S.active) return;
+   SlabView v;
+   slab_view(S, a.parity, a.dt, v);
+   cell_mixed(a, S, v, blk, lane);
+   if (blk == S.first_blk && lane == 0) slab_advance(S, a.parity, v);
+}
+
+template __global__ void k_engine_mixed<3>(EngineArgs);
+template __global__ void k_engine_mixed<4>(EngineArgs);
+template __global__ void k_engine_mixed<5>(EngineArgs);
+
+}  // namespace cdh

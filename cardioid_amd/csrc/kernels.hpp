@@ -43,24 +43,6 @@ struct StencilArgs {
    double *dVm;
 };
 
-// Stimulus values of one step: value[k] applies where bit k of a cell's
-// mask is set and bit k of `on` is set; later stimuli overwrite earlier
-// ones, as BoxStimulus assigns (elec/BoxStimulus.cc:61-62).
-struct StimStep {
-   int nstim;
-   uint32_t on;
-   double value[kMaxStim];
-};
-
-__device__ __forceinline__ double stim_value(const StimStep &s, uint8_t mask)
-{
-   double v = 0.0;
-#pragma unroll
-   for (int k = 0; k < kMaxStim; k++)
-      if (k < s.nstim && ((mask >> k) & 1u) && ((s.on >> k) & 1u)) v = s.value[k];
-   return v;
-}
-
 // Activation sensors, evaluated after the step at the new time.
 //   tact0: first time Vm > 0            (ActivationTimeSensor.cc:99-111)
 //   tact / trec: first up-crossing of the threshold, and the first
@@ -145,66 +127,6 @@ __device__ __forceinline__ void sensor_eval_pair(const SensorArgs &a, int i, dou
    if (a.apd) apd_eval(a, i, v1, vrest, now);
 }
 
-struct FusedArgs {
-   TT06Const K;
-   const double *coef;
-   const ParamSet *params;
-   const uint8_t *pid;
-   const uint8_t *flags;
-   double *state;
-   int64_t stride;
-   int n;
-   int nblock_per_xcd;
-   const uint32_t *bi;
-   const double *box_cur;
-   double *box_next;
-   const double *A0;
-   const float *Aw;
-   int off[19];
-   double scale;
-   double dt;
-   uint32_t gate_mask;          // all 12 bits set; see gates64
-   int stim_any;
-   const uint8_t *stim_mask;
-   StimStep stim;
-   SensorArgs sens;
-};
-
-struct MixedArgs {
-   TT06Const K;
-   const double *coef;
-   const ParamSet *params;
-   const uint8_t *pid;
-   const uint8_t *flags;
-   float *st32;                 // [17][stride]
-   double *st64;                // [2][stride]: Ca_i, Na_i
-   int64_t stride;
-   int n;
-   int nblock_per_xcd;
-   const uint32_t *bi;
-   const double *box_cur;
-   double *box_next;
-   const float *Aw;
-   int off[19];
-   double scale;
-   double dt;
-   uint32_t gate_mask;
-   int stim_any;
-   const uint8_t *stim_mask;
-   StimStep stim;
-   SensorArgs sens;
-};
-
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
-// XCD owns one contiguous range of cells (and of the box), which keeps the
-// stencil's neighbour rows in that XCD's L2.
-__device__ __forceinline__ int xcd_cell_index(int n, int nblock_per_xcd)
-{
-   const int wg = blockIdx.x;
-   const int logical = (wg & 7) * nblock_per_xcd + (wg >> 3);
-   return logical * blockDim.x + threadIdx.x;
-}
-
 // kernels (kernels_f64.hip)
 template <bool BLK>
 __global__ void k_tt06_calc_f64(ReactionArgs a);
@@ -217,10 +139,5 @@ __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR
 __global__ void k_fill_f64(int n, double v, double *x);
 __global__ void k_set_list_f64(int n, const int *list, double v, double *x);
 __global__ void k_sensor_f64(SensorArgs a, const double *Vm, const double *vrest);
-template <int WAVES>
-__global__ void k_step_fused_f64(FusedArgs a);
-// kernels (kernels_mixed.hip)
-template <int WAVES>
-__global__ void k_step_fused_mixed(MixedArgs a);
 
 }  // namespace cdh

@@ -9,7 +9,6 @@
 //   k_integrate_f64     Vm += dt*(dVmR+dVmD+iStim)      elec/simulationLoop.cc:221-227
 //   k_fill_f64 / k_set_list_f64   iStim = 0; BoxStimulus  elec/simulationLoop.cc:186-193,
 //                                                   elec/BoxStimulus.cc:53-64
-//   k_step_fused_f64    all of the above for one omp-loop step, one launch
 //   k_sensor_f64        ActivationTimeSensor / ActivationAndRecoverySensor eval
 #pragma clang fp contract(off)
 
@@ -133,48 +132,5 @@ __global__ __launch_bounds__(256) void k_sensor_f64(SensorArgs a, const double *
    if (i >= a.n) return;
    sensor_eval(a, i, Vm[i], a.apd ? vrest[i] : 0.0);
 }
-
-// One omp-loop step for every tissue cell, Vm in the padded box layout and
-// double buffered: reads box_cur, writes box_next.  Per cell, in order:
-// stimulus value, reaction at Vm_t, 19-point stencil on Vm_t, integrator.
-// WAVES is the occupancy the register allocation targets (waves per SIMD).
-template <int WAVES>
-__global__ __launch_bounds__(256, WAVES) void k_step_fused_f64(FusedArgs a)
-{
-   const int i = xcd_cell_index(a.n, a.nblock_per_xcd);
-   if (i >= a.n) return;
-   const uint32_t b = a.bi[i];
-   const double *phi = a.box_cur + b;
-   const double V = phi[0];
-
-   // states and weights stream once per step: 64-cell blocks, non-temporal
-   double *sp = a.state + blk_base(i, N_STATE);
-   double st[N_STATE];
-#pragma unroll
-   for (int s = 0; s < N_STATE; s++) st[s] = __builtin_nontemporal_load(sp + kBlk * s);
-   const ParamSet P = a.params[a.pid[i]];
-   const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
-   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
-#pragma unroll
-   for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
-
-   const float *wp = a.Aw + blk_base(i, 18);
-   double tmp = __builtin_nontemporal_load(a.A0 + i) * phi[0];
-#pragma unroll
-   for (int k = 1; k < 19; k++)
-      tmp += (double)__builtin_nontemporal_load(wp + kBlk * (k - 1)) * phi[a.off[k]];
-   const double dVD = tmp * a.scale;
-
-   double iStim = 0.0;
-   if (a.stim_any) iStim = stim_value(a.stim, a.stim_mask[i]);
-
-   const double Vn = V + a.dt * (dVR + dVD + iStim);
-   a.box_next[b] = Vn;
-   if (a.sens.enabled) sensor_eval_pair(a.sens, i, V, Vn, P.Vm0);
-}
-
-template __global__ void k_step_fused_f64<2>(FusedArgs);
-template __global__ void k_step_fused_f64<3>(FusedArgs);
-template __global__ void k_step_fused_f64<4>(FusedArgs);
 
 }  // namespace cdh

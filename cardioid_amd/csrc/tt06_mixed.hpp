@@ -41,13 +41,24 @@ __device__ __forceinline__ double fma_vvs(double x, double s, double a)
    return r;
 }
 
+// Horner with FMA.  The first step is a multiply and an add, which take
+// their coefficient straight from SGPRs; a first FMA would need one of its
+// two coefficients copied into VGPRs (one scalar operand per instruction).
 template <int LO, int HI>
 __device__ __forceinline__ double hornerFma(const double *__restrict__ a, double x)
 {
-   double s = a[HI];
+   if constexpr (HI == LO) {
+      return a[HI];
+   } else {
+      double s;
+      {
+#pragma clang fp contract(off)
+         s = x * a[HI] + a[HI - 1];
+      }
 #pragma unroll
-   for (int j = HI - 1; j >= LO; --j) s = fma_vvs(x, s, a[j]);
-   return s;
+      for (int j = HI - 2; j >= LO; --j) s = fma_vvs(x, s, a[j]);
+      return s;
+   }
 }
 
 __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }

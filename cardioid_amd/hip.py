@@ -66,6 +66,21 @@ class SimParams(C.Structure):
                 ("gate_tail_rule", C.c_int), ("sensors", C.c_int), ("variant", C.c_int)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("dt", C.c_double), ("fit_l", ip), ("fit_m", ip),
+                ("fit_coef", dp), ("cell_capacity", C.c_int64), ("box_capacity", C.c_int64),
+                ("slab_capacity", C.c_int), ("param_capacity", C.c_int), ("sensors", C.c_int)]
+
+
+class SlabDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double),
+                ("dy", C.c_double), ("dz", C.c_double), ("diffusionScale", C.c_double),
+                ("ncell", C.c_int64), ("gid", i64p), ("celltype", u8p), ("sigma", dp),
+                ("ncelltype", C.c_int), ("celltypes", C.POINTER(CellType)), ("scales", dp),
+                ("threshold", C.c_double), ("gate_tail_rule", C.c_int), ("nstim", C.c_int),
+                ("stim", C.POINTER(BoxStim))]
+
+
 EXPORTS = [
     "cdh_last_error", "cdh_version", "cdh_celltype_standard",
     "cdh_tt06_create", "cdh_tt06_destroy", "cdh_tt06_calc", "cdh_tt06_update_nongate",
@@ -81,7 +96,14 @@ EXPORTS = [
     "cdh_sim_get_vm", "cdh_sim_get_state", "cdh_sim_set_state", "cdh_sim_get_activation",
     "cdh_sim_get_t90",
     "cdh_sim_time", "cdh_sim_loop", "cdh_sim_info",
+    "cdh_batch_create", "cdh_batch_destroy", "cdh_slab_extent", "cdh_batch_add",
+    "cdh_batch_remove", "cdh_batch_run", "cdh_batch_run_timed", "cdh_batch_sync",
+    "cdh_batch_set_graph", "cdh_batch_set_launch", "cdh_batch_get_launch", "cdh_batch_autotune",
+    "cdh_batch_get_vm", "cdh_batch_get_state", "cdh_batch_get_activation", "cdh_batch_get_t90",
+    "cdh_batch_slab_info", "cdh_batch_info",
 ]
+
+ERR_FULL = 5
 
 _lib = None
 
@@ -148,6 +170,24 @@ def lib():
         L.cdh_sim_get_t90.argtypes = [C.c_void_p, dp]
         L.cdh_sim_info.argtypes = [C.c_void_p, i64p, ip, ip, ip]
         L.cdh_celltype_standard.argtypes = [C.c_char_p, C.POINTER(CellType)]
+        L.cdh_batch_create.argtypes = [C.POINTER(BatchParams), C.POINTER(C.c_void_p)]
+        L.cdh_batch_destroy.argtypes = [C.c_void_p]
+        L.cdh_slab_extent.argtypes = [C.POINTER(SlabDesc), i64p, i64p, ip]
+        L.cdh_batch_add.argtypes = [C.c_void_p, C.POINTER(SlabDesc), ip]
+        L.cdh_batch_remove.argtypes = [C.c_void_p, C.c_int]
+        L.cdh_batch_run.argtypes = [C.c_void_p, C.c_int]
+        L.cdh_batch_run_timed.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cdh_batch_sync.argtypes = [C.c_void_p]
+        L.cdh_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.cdh_batch_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.cdh_batch_get_launch.argtypes = [C.c_void_p, ip, ip]
+        L.cdh_batch_autotune.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_int, ip]
+        L.cdh_batch_get_vm.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cdh_batch_get_state.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cdh_batch_get_activation.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
+        L.cdh_batch_get_t90.argtypes = [C.c_void_p, C.c_int, dp]
+        L.cdh_batch_slab_info.argtypes = [C.c_void_p, C.c_int, i64p, dp, ip]
+        L.cdh_batch_info.argtypes = [C.c_void_p, i64p, i64p, i64p, ip]
         _lib = L
     return _lib
 
@@ -271,3 +311,151 @@ class Simulation:
         check(lib().cdh_sim_info(self.h, C.byref(b), C.byref(bx), C.byref(by), C.byref(bz)),
               "cdh_sim_info")
         return dict(device_bytes=b.value, box=(bx.value, by.value, bz.value))
+
+
+class Slab:
+    """One slab description for the batched engine (cdh_slab_desc): an
+    anatomy with its cell types, stimuli and per-cell drug scales."""
+
+    def __init__(self, anatomy, celltypes: Sequence[str], diffusion_scale: float, stimuli=(),
+                 scales=None, threshold: float = -40.0, gate_tail_rule: bool = True):
+        cts = (CellType * len(celltypes))(*[standard_celltype(n) for n in celltypes])
+        self.ncell = int(anatomy.gid.size)
+        gid = np.ascontiguousarray(anatomy.gid, dtype=np.int64)
+        ctype = np.ascontiguousarray(anatomy.celltype, dtype=np.uint8)
+        sigma = np.ascontiguousarray(anatomy.sigma, dtype=np.float64).reshape(-1)
+        sc = None
+        if scales is not None:
+            sc = np.asarray(scales, dtype=np.float64)
+            if sc.size == 3:
+                sc = np.tile(sc, self.ncell)
+            sc = np.ascontiguousarray(sc.reshape(-1))
+        st = (BoxStim * max(1, len(stimuli)))(*stimuli)
+        self._keep = [cts, gid, ctype, sigma, sc, st]
+        self.desc = SlabDesc(nx=anatomy.nx, ny=anatomy.ny, nz=anatomy.nz, dx=anatomy.dx,
+                             dy=anatomy.dy, dz=anatomy.dz, diffusionScale=diffusion_scale,
+                             ncell=self.ncell, gid=ptr(gid, i64p), celltype=ptr(ctype, u8p),
+                             sigma=ptr(sigma), ncelltype=len(celltypes), celltypes=cts,
+                             scales=ptr(sc) if sc is not None else None, threshold=threshold,
+                             gate_tail_rule=int(gate_tail_rule), nstim=len(stimuli), stim=st)
+
+    def extent(self):
+        """(cells, box points, parameter sets) the slab takes in an arena."""
+        c, b, p = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().cdh_slab_extent(C.byref(self.desc), C.byref(c), C.byref(b), C.byref(p)),
+              "cdh_slab_extent")
+        return c.value, b.value, p.value
+
+
+class ArenaFull(CardioidHipError):
+    pass
+
+
+class Batch:
+    """Batched engine over the C ABI (cdh_batch_*): slabs advanced together,
+    one launch per step, optionally replayed from captured HIP graphs."""
+
+    def __init__(self, dt: float, cell_capacity: int, box_capacity: int, slab_capacity: int = 256,
+                 param_capacity: int = 4096, mode: str = "mixed_fused", sensors: int = 1,
+                 fits: Optional[FitSet] = None):
+        self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
+        fl, fm, fc = self.fits.packed()
+        self._keep = [fl, fm, fc]
+        p = BatchParams(mode=MODES[mode], dt=dt, fit_l=ptr(fl, ip), fit_m=ptr(fm, ip),
+                        fit_coef=ptr(fc), cell_capacity=int(cell_capacity),
+                        box_capacity=int(box_capacity), slab_capacity=int(slab_capacity),
+                        param_capacity=int(param_capacity), sensors=int(sensors))
+        h = C.c_void_p()
+        check(lib().cdh_batch_create(C.byref(p), C.byref(h)), "cdh_batch_create")
+        self.h = h
+        self.dt = dt
+        self.ncell = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, slab: Slab) -> int:
+        sid = C.c_int()
+        rc = lib().cdh_batch_add(self.h, C.byref(slab.desc), C.byref(sid))
+        if rc == ERR_FULL:
+            raise ArenaFull("arena full")
+        check(rc, "cdh_batch_add")
+        self.ncell[sid.value] = slab.ncell
+        return sid.value
+
+    def remove(self, sid: int):
+        check(lib().cdh_batch_remove(self.h, sid), "cdh_batch_remove")
+        self.ncell.pop(sid, None)
+
+    def run(self, nsteps: int):
+        check(lib().cdh_batch_run(self.h, int(nsteps)), "cdh_batch_run")
+
+    def run_timed(self, nsteps: int) -> float:
+        ms = C.c_double()
+        check(lib().cdh_batch_run_timed(self.h, int(nsteps), C.byref(ms)), "cdh_batch_run_timed")
+        return ms.value
+
+    def sync(self):
+        check(lib().cdh_batch_sync(self.h), "cdh_batch_sync")
+
+    def set_graph(self, steps: int):
+        check(lib().cdh_batch_set_graph(self.h, int(steps)), "cdh_batch_set_graph")
+
+    def set_launch(self, waves: int, block: int):
+        check(lib().cdh_batch_set_launch(self.h, waves, block), "cdh_batch_set_launch")
+
+    def launch(self):
+        w, b = C.c_int(), C.c_int()
+        check(lib().cdh_batch_get_launch(self.h, C.byref(w), C.byref(b)), "cdh_batch_get_launch")
+        return w.value, b.value
+
+    def autotune(self, steps: int = 10):
+        best = C.c_double()
+        table = np.zeros(3 * 64)
+        nrows = C.c_int()
+        check(lib().cdh_batch_autotune(self.h, int(steps), C.byref(best), ptr(table), 64,
+                                       C.byref(nrows)), "cdh_batch_autotune")
+        rows = [(int(table[3 * k]), int(table[3 * k + 1]), float(table[3 * k + 2]))
+                for k in range(min(nrows.value, 64))]
+        return best.value, rows
+
+    def vm(self, sid: int) -> np.ndarray:
+        out = np.zeros(self.ncell[sid])
+        check(lib().cdh_batch_get_vm(self.h, sid, ptr(out)), "cdh_batch_get_vm")
+        return out
+
+    def state(self, sid: int) -> np.ndarray:
+        out = np.zeros(NSTATE * self.ncell[sid])
+        check(lib().cdh_batch_get_state(self.h, sid, ptr(out)), "cdh_batch_get_state")
+        return out.reshape(NSTATE, -1)
+
+    def activation(self, sid: int):
+        a0, a, r = (np.zeros(self.ncell[sid]) for _ in range(3))
+        check(lib().cdh_batch_get_activation(self.h, sid, ptr(a0), ptr(a), ptr(r)),
+              "cdh_batch_get_activation")
+        return a0, a, r
+
+    def t90(self, sid: int) -> np.ndarray:
+        out = np.zeros(self.ncell[sid])
+        check(lib().cdh_batch_get_t90(self.h, sid, ptr(out)), "cdh_batch_get_t90")
+        return out
+
+    def slab_info(self, sid: int):
+        n, t, l = C.c_int64(), C.c_double(), C.c_int()
+        check(lib().cdh_batch_slab_info(self.h, sid, C.byref(n), C.byref(t), C.byref(l)),
+              "cdh_batch_slab_info")
+        return dict(ncell=n.value, time=t.value, loop=l.value)
+
+    def info(self):
+        b, c, x, n = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().cdh_batch_info(self.h, C.byref(b), C.byref(c), C.byref(x), C.byref(n)),
+              "cdh_batch_info")
+        return dict(device_bytes=b.value, cells_used=c.value, box_used=x.value, nslab=n.value)

@@ -164,3 +164,67 @@ def endocardial_point_stimuli(anat: Anatomy, depth: np.ndarray, n: int = 8, seed
         out.append(box_stimulus(x[c] - half, x[c] + half, y[c] - half, y[c] + half,
                                 z[c] - half, z[c] + half, vStim=vStim, duration=duration))
     return out
+
+
+def fibre_slab(nx: int, ny: int, nz: int, phi_deg: float, dx: float = 0.2,
+               celltype: int = 0) -> Anatomy:
+    """All-tissue slab with one fibre direction in the x-y plane at angle phi
+    from x (the `fibre` conductivity with theta = 90 deg,
+    elec/FibreConductivity.cc:19-62)."""
+    n = nx * ny * nz
+    p = np.deg2rad(phi_deg)
+    f = np.tile(np.array([np.cos(p), np.sin(p), 0.0]), (n, 1))
+    return Anatomy(nx, ny, nz, dx, dx, dx, np.arange(n, dtype=np.int64),
+                   np.full(n, celltype, np.uint8), fibre_tensor(f))
+
+
+@dataclass
+class SlabCase:
+    anatomy: Anatomy
+    scales: np.ndarray      # (3,): s_kr, s_cal, s_na
+    stimuli: list
+    phi_deg: float
+
+
+def _log_uniform_dims(rng, lo, hi):
+    return tuple(int(round(np.exp(rng.uniform(np.log(a), np.log(b))))) for a, b in zip(lo, hi))
+
+
+def batched_slabs(n: int = 64, seed: int = 1, lo=(32, 32, 32), hi=(160, 160, 96)) -> List[SlabCase]:
+    """Independent tissue slabs for the batched engine: each axis drawn
+    log-uniform between lo and hi, drug scales (s_kr, s_cal, s_na) uniform in
+    [0, 1], fibre angle uniform in [-90, 90] deg, endocardial cells, and the
+    box stimulus x < 4 (-52 uA/uF for 1 ms at t = 0)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx, ny, nz = _log_uniform_dims(rng, lo, hi)
+        phi = float(rng.uniform(-90.0, 90.0))
+        sc = rng.uniform(0.0, 1.0, 3)
+        a = fibre_slab(nx, ny, nz, phi)
+        out.append(SlabCase(a, sc, [box_stimulus(xMax=4, nx=nx, ny=ny, nz=nz)], phi))
+    return out
+
+
+@dataclass
+class Request:
+    arrival_s: float        # wall-clock arrival after the stream starts
+    dims: Tuple[int, int, int]
+    duration_ms: float
+    scales: np.ndarray
+    phi_deg: float
+
+
+def request_stream(n: int, rate_hz: float = 20.0, seed: int = 4, lo: int = 32, hi: int = 128,
+                   dur_ms=(200.0, 1000.0)) -> List[Request]:
+    """Drug-screen requests with Poisson arrivals: each axis log-uniform in
+    [lo, hi], duration uniform in dur_ms, scales uniform in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate_hz)
+        dims = _log_uniform_dims(rng, (lo,) * 3, (hi,) * 3)
+        out.append(Request(t, dims, float(rng.uniform(*dur_ms)), rng.uniform(0.0, 1.0, 3),
+                           float(rng.uniform(-90.0, 90.0))))
+    return out

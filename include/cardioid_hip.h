@@ -43,6 +43,7 @@ extern "C" {
 #define CDH_ERR_HIP 2
 #define CDH_ERR_UNSUPPORTED 3
 #define CDH_ERR_STATE 4
+#define CDH_ERR_FULL 5         /* the engine's arena has no room for the slab */
 
 #define CDH_NSTATE 19
 #define CDH_NFIT 33
@@ -219,6 +220,79 @@ double cdh_sim_time(const cdh_sim *s);
 int cdh_sim_loop(const cdh_sim *s);
 /* bytes of device memory held, and the box dimensions used */
 int cdh_sim_info(const cdh_sim *s, int64_t *device_bytes, int *bx, int *by, int *bz);
+
+
+/* ------------------------------------------------------------------------ */
+/* Batched engine: many slabs advanced together, one launch per step          */
+/* ------------------------------------------------------------------------ */
+/* No single counterpart in the reference: each slab is what one Cardioid
+ * run (simulationLoop, elec/simulationLoop.cc:150-243, on one anatomy) would
+ * compute; the engine advances any number of them together.  Slabs keep
+ * their own clock (time += dt from 0 when added), stimuli, drug scales and
+ * sensors; their results do not depend on what else is in the batch.  Device
+ * memory is one arena sized at creation (cells, box points, slabs, parameter
+ * sets); slabs can be added and removed between runs. */
+typedef struct cdh_batch cdh_batch;
+
+typedef struct {
+   int mode;                    /* CDH_MODE_FP64_FUSED or CDH_MODE_MIXED_FUSED */
+   double dt;
+   const int *fit_l;
+   const int *fit_m;
+   const double *fit_coef;
+   int64_t cell_capacity;       /* cells; each slab takes ceil(ncell/64)*64 */
+   int64_t box_capacity;        /* box points; a slab takes (bx*by*bz) of its padded box */
+   int slab_capacity;
+   int param_capacity;          /* parameter sets; a slab takes one per (cell type, scales) */
+   int sensors;                 /* bit 0: activation times; bit 1: APD90 as well */
+} cdh_batch_params;
+
+typedef struct {
+   int nx, ny, nz;
+   double dx, dy, dz, diffusionScale;
+   int64_t ncell;
+   const int64_t *gid;          /* ncell */
+   const uint8_t *celltype;     /* ncell, index into celltypes */
+   const double *sigma;         /* 6*ncell */
+   int ncelltype;               /* <= 8 */
+   const cdh_celltype *celltypes;
+   const double *scales;        /* 3*ncell (s_kr, s_cal, s_na) or NULL */
+   double threshold;            /* activation threshold, mV */
+   int gate_tail_rule;
+   int nstim;                   /* <= 8 */
+   const cdh_box_stim *stim;
+} cdh_slab_desc;
+
+int cdh_batch_create(const cdh_batch_params *p, cdh_batch **out);
+int cdh_batch_destroy(cdh_batch *b);
+/* arena units a slab needs: cells (whole tiles of 64), box points, parameter sets */
+int cdh_slab_extent(const cdh_slab_desc *d, int64_t *cells, int64_t *box_points, int *param_sets);
+/* adds a slab; CDH_ERR_FULL if the arena has no room for it now */
+int cdh_batch_add(cdh_batch *b, const cdh_slab_desc *d, int *slab);
+int cdh_batch_remove(cdh_batch *b, int slab);
+/* advances every slab by nsteps; asynchronous on the batch's stream */
+int cdh_batch_run(cdh_batch *b, int nsteps);
+int cdh_batch_run_timed(cdh_batch *b, int nsteps, double *ms);
+int cdh_batch_sync(cdh_batch *b);
+/* steps per captured HIP graph (even; 0 launches each step directly) */
+int cdh_batch_set_graph(cdh_batch *b, int steps);
+/* kernel occupancy target (waves/SIMD: FP64 2..4, mixed 3..5) and workgroup
+ * size (64, 128, 256) */
+int cdh_batch_set_launch(cdh_batch *b, int waves, int block);
+int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block);
+/* times every launch configuration for `steps` steps from the current state
+ * (which is restored afterwards) and keeps the fastest; the table gets
+ * (waves, block, ms per step) rows, up to max_rows */
+int cdh_batch_autotune(cdh_batch *b, int steps, double *best_ms, double *table, int max_rows,
+                       int *nrows);
+/* per-slab results, in the slab's input cell order */
+int cdh_batch_get_vm(cdh_batch *b, int slab, double *vm);
+int cdh_batch_get_state(cdh_batch *b, int slab, double *state);      /* [19][ncell] */
+int cdh_batch_get_activation(cdh_batch *b, int slab, double *tact0, double *tact, double *trec);
+int cdh_batch_get_t90(cdh_batch *b, int slab, double *t90);
+int cdh_batch_slab_info(cdh_batch *b, int slab, int64_t *ncell, double *time, int *loop);
+int cdh_batch_info(cdh_batch *b, int64_t *device_bytes, int64_t *cells_used, int64_t *box_used,
+                   int *nslab);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,198 @@
+"""The batched engine (cdh_batch_*, through the C ABI) on an MI355X.
+
+- FP64 slabs in one batch each match the oracle run of that slab alone (the
+  drop-in bar: Vm within 1e-9 relative after the run, activation identical).
+- Every slab's result is the same alone and in a batch, bit for bit, in both
+  precisions, with slabs added and removed while others run.
+- Steps replayed from captured HIP graphs equal directly launched steps.
+- The autotuner leaves the simulation where it found it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cardioid_amd import hip, workloads as W
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+DT = 0.02
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    hip.lib()
+    yield
+
+
+def relerr(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+
+
+def small_cases():
+    """Three unlike slabs: a brick with a box stimulus, a shell with three
+    cell types and point stimuli, and a fibre slab under drug block.
+    Returns (anatomy, cell types, stimuli, per-cell scales or None, Slab)."""
+    a1 = W.brick(10, 8, 6)
+    st1 = [W.box_stimulus(xMax=3, nx=10, ny=8, nz=6)]
+    a2, depth = W.ellipsoid_shell(16, 14, 20, seed=2)
+    st2 = W.endocardial_point_stimuli(a2, depth, 2, seed=2)
+    a3 = W.fibre_slab(9, 7, 5, 30.0)
+    sc3 = np.array([0.5, 0.7, 0.9])
+    st3 = [W.box_stimulus(xMax=4, nx=9, ny=7, nz=5)]
+    return [(a1, ["endoCellML"], st1, None,
+             hip.Slab(a1, ["endoCellML"], W.DIFFUSION_SCALE, st1)),
+            (a2, W.CELLTYPES_EMI, st2, None,
+             hip.Slab(a2, W.CELLTYPES_EMI, W.DIFFUSION_SCALE, st2)),
+            (a3, ["endoCellML"], st3, np.tile(sc3, a3.ncell),
+             hip.Slab(a3, ["endoCellML"], W.DIFFUSION_SCALE, st3, scales=sc3))]
+
+
+def capacity(slabs):
+    cells = sum(s.extent()[0] for s in slabs)
+    box = sum(s.extent()[1] for s in slabs)
+    return cells, box
+
+
+def test_fp64_batch_matches_oracle_per_slab():
+    import oracle_lib as OL
+    cases = small_cases()
+    slabs = [c[4] for c in cases]
+    cells, box = capacity(slabs)
+    nsteps = 600
+    say(f"fp64 batch of {len(slabs)} slabs, {nsteps} steps, graphs of 4 steps")
+    b = hip.Batch(DT, cells, box, slab_capacity=8, mode="fp64_fused")
+    b.set_graph(4)
+    ids = [b.add(s) for s in slabs]
+    b.run(nsteps)
+    for (a, cts, stims, sc, _), sid in zip(cases, ids):
+        ref = OL.OracleSim(a, cts, DT, W.DIFFUSION_SCALE, stims, scales=sc)
+        ref.run(nsteps)
+        e = relerr(b.vm(sid), ref.vm())
+        act, want = b.activation(sid), ref.activation()
+        info = b.slab_info(sid)
+        say(f"  slab {sid}: {a.ncell} cells, Vm max rel {e.max():.2e}, "
+            f"activated {(act[1] >= 0).mean():.3f}, time {info['time']:.4f}")
+        assert info["time"] == ref.time and info["loop"] == nsteps
+        assert e.max() < 1e-9
+        for k in range(3):
+            assert np.array_equal(act[k], want[k])
+        ref.close()
+    b.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64_fused", "mixed_fused"])
+def test_alone_equals_in_batch_with_arrivals_and_departures(mode):
+    """Slab C joins after 150 steps while A and B run; B leaves at 300 and D
+    takes its place.  Each slab's Vm, states and activation times equal those
+    of the same slab run alone for the same number of its own steps."""
+    cases = small_cases()
+    sa, sb, sc = cases[0][4], cases[1][4], cases[2][4]
+    a4 = W.fibre_slab(12, 6, 4, -45.0)
+    sd = hip.Slab(a4, ["epiCellML"], W.DIFFUSION_SCALE, [W.box_stimulus(xMax=2, nx=12, ny=6, nz=4)],
+                  scales=np.array([0.2, 1.0, 0.6]))
+    cells, box = capacity([sa, sb, sc, sd])
+    say(f"{mode}: arrivals and departures in one batch")
+    b = hip.Batch(DT, cells, box, slab_capacity=4, mode=mode, sensors=3)
+    b.set_graph(6)
+    ia, ib = b.add(sa), b.add(sb)
+    b.run(150)
+    ic = b.add(sc)
+    b.run(150)
+    got_b = (b.vm(ib), b.state(ib), b.activation(ib))
+    b.remove(ib)
+    idd = b.add(sd)
+    b.run(401)
+    got = {"a": (b.vm(ia), b.state(ia), b.activation(ia), b.t90(ia), 701),
+           "c": (b.vm(ic), b.state(ic), b.activation(ic), b.t90(ic), 551),
+           "d": (b.vm(idd), b.state(idd), b.activation(idd), b.t90(idd), 401)}
+    b.close()
+
+    def alone(slab, n):
+        c, x = capacity([slab])
+        one = hip.Batch(DT, c, x, slab_capacity=1, mode=mode, sensors=3)
+        s = one.add(slab)
+        one.run(n)
+        out = (one.vm(s), one.state(s), one.activation(s), one.t90(s))
+        one.close()
+        return out
+
+    ref_b = alone(sb, 300)
+    assert np.array_equal(got_b[0], ref_b[0]) and np.array_equal(got_b[1], ref_b[1])
+    for k in range(3):
+        assert np.array_equal(got_b[2][k], ref_b[2][k])
+    for name, slab in (("a", sa), ("c", sc), ("d", sd)):
+        g = got[name]
+        r = alone(slab, g[4])
+        same = [np.array_equal(g[0], r[0]), np.array_equal(g[1], r[1]),
+                all(np.array_equal(g[2][k], r[2][k]) for k in range(3)), np.array_equal(g[3], r[3])]
+        say(f"  slab {name}: {g[4]} steps, identical (Vm, states, activation, t90) = {same}")
+        assert all(same)
+
+
+def test_graph_replay_equals_direct_launches():
+    cases = small_cases()
+    slabs = [c[4] for c in cases]
+    cells, box = capacity(slabs)
+    out = []
+    for graph in (0, 2, 10):
+        b = hip.Batch(DT, cells, box, slab_capacity=4, mode="mixed_fused")
+        b.set_graph(graph)
+        ids = [b.add(s) for s in slabs]
+        for chunk in (7, 40, 33, 120):     # odd chunks: direct steps between replays
+            b.run(chunk)
+            b.sync()
+            _ = b.vm(ids[0])
+        out.append([np.concatenate([b.vm(i), b.state(i).ravel(), b.activation(i)[1]]) for i in ids])
+        b.close()
+    for g in (1, 2):
+        for k in range(len(slabs)):
+            assert np.array_equal(out[0][k], out[g][k])
+    say("  graphs of 2 and 10 steps replay the direct launches bit for bit")
+
+
+def test_autotune_restores_state_and_picks_a_candidate():
+    cases = small_cases()
+    slabs = [c[4] for c in cases]
+    cells, box = capacity(slabs)
+    res = []
+    for tune in (False, True):
+        b = hip.Batch(DT, cells, box, slab_capacity=4, mode="mixed_fused", sensors=3)
+        ids = [b.add(s) for s in slabs]
+        b.run(50)
+        if tune:
+            best, rows = b.autotune(5)
+            say(f"  autotune: best {best:.4f} ms/step, launch {b.launch()}, {len(rows)} candidates")
+            assert len(rows) == 9 and best > 0
+            assert b.launch() in [(r[0], r[1]) for r in rows]
+        b.run(250)
+        res.append([np.concatenate([b.vm(i), b.state(i).ravel(), b.t90(i)]) for i in ids])
+        assert [b.slab_info(i)["loop"] for i in ids] == [300] * 3
+        b.close()
+    for k in range(len(slabs)):
+        assert np.array_equal(res[0][k], res[1][k])
+
+
+def test_arena_full_and_reuse():
+    cases = small_cases()
+    s1, s2 = cases[0][4], cases[1][4]
+    c1, x1 = capacity([s1])
+    c2, x2 = capacity([s2])
+    b = hip.Batch(DT, max(c1, c2), max(x1, x2), slab_capacity=2, mode="mixed_fused")
+    i1 = b.add(s1)
+    with pytest.raises(hip.ArenaFull):
+        b.add(s2)
+    b.run(10)
+    b.remove(i1)
+    i2 = b.add(s2)
+    b.run(10)
+    assert b.slab_info(i2)["loop"] == 10
+    assert b.info()["nslab"] == 1
+    b.close()
