@@ -43,6 +43,9 @@ SlabInput slab_input(const cdh_slab_desc *d, std::vector<CellType> &cts)
    in.gate_tail_rule = d->gate_tail_rule;
    in.nstim = d->nstim;
    in.stim = d->stim;
+   in.check_ranges = d->check_ranges;
+   in.vm_min = d->vm_min;
+   in.vm_max = d->vm_max;
    return in;
 }
 
@@ -231,6 +234,22 @@ int cdh_batch_get_t90(cdh_batch *b, int slab, double *t90)
       std::vector<double> v(s.n);
       e.t90(slab, v.data());
       scatter(s, v.data(), t90);
+   });
+}
+
+int cdh_batch_range_report(cdh_batch *b, int slab, int64_t *count, int *loop, int64_t *cell,
+                           double *V, double *dVmR, double *dVmD)
+{
+   return guard([&] {
+      Engine &e = engine(b);
+      const RangeRec r = e.range_report(slab);
+      if (count) *count = (int64_t)r.count;
+      if (!r.taken) return;
+      if (loop) *loop = r.loop;
+      if (cell) *cell = e.slab(slab).perm[r.cell];
+      if (V) *V = r.V;
+      if (dVmR) *dVmR = r.dVmR;
+      if (dVmD) *dVmD = r.dVmD;
    });
 }
 

@@ -59,9 +59,11 @@ struct alignas(16) SlabDev {
    int32_t param_base;          // parameter sets of this slab start here
    int32_t nstim;
    uint32_t stim_on[2];         // bit k: stimulus k is nonzero at time[p]
-   int32_t pad_[2];
+   int32_t check_range;         // Vm range check on (Simulate::checkRanges)
+   int32_t pad_;
    double scale;                // diffusionScale
    double threshold;            // activation threshold
+   double vm_min, vm_max;       // the check's range
    double stim_val[2][kSlabStim];
    PulseDev stim[kSlabStim];
 };
@@ -92,6 +94,7 @@ struct EngineArgs {
    double *tact0, *tact, *trec;
    uint8_t *apd_phase;
    double *vmax, *t90;
+   RangeRec *ranges;            // per slab
 };
 
 // PeriodicPulse::eval at `time` inside the window t0 < time < tf, else 0
@@ -182,6 +185,18 @@ __device__ __forceinline__ void engine_sensors(const EngineArgs &a, const SlabVi
    s.vmax = a.vmax;
    s.t90 = a.t90;
    sensor_eval_pair(s, (int)i, v0, v1, vrest);
+}
+
+// Vm range check of a wavefront (cells whose Vm at the step's start is out
+// of range); uniform branch, one atomic per wavefront that finds any.
+__device__ __forceinline__ void engine_range(const EngineArgs &a, const SlabDev &S, int sl,
+                                             int parity, bool live, int lane, int64_t i, double V,
+                                             double dVR, double dVD)
+{
+   if (!S.check_range) return;
+   const bool out = live && (V > S.vm_max || V < S.vm_min);
+   const uint64_t m = __ballot(out);
+   if (m) range_report(a.ranges + sl, m, lane, S.loop[parity], i, V, dVR, dVD);
 }
 
 // Buffer resource over a box so that (box index)*8 is the VGPR offset and

@@ -13,8 +13,8 @@ namespace cdh {
 
 namespace {
 
-__device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, const SlabView &v,
-                                         int blk, int lane)
+__device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, int sl,
+                                         const SlabView &v, int blk, int lane)
 {
    const int64_t i = (int64_t)blk * kBlk + lane;
    // padding cells of a slab's last tile compute on a copy of a tissue
@@ -45,6 +45,7 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    for (int k = 1; k < 19; k++)
       tmp += (double)__builtin_nontemporal_load(wp + kBlk * (k - 1)) * box_load(R, b8, S.soff[k]);
    const double dVD = tmp * v.scale;
+   engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);
    if (live) {
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
    if (!S.active) return;
    SlabView v;
    slab_view(S, a.parity, a.dt, v);
-   cell_f64(a, S, v, blk, lane);
+   cell_f64(a, S, sl, v, blk, lane);
    if (blk == S.first_blk && lane == 0) slab_advance(S, a.parity, v);
 }
 

@@ -193,6 +193,8 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    CDH_HIP(hipMemset(blk_slab_.p, 0xFF, ntile * sizeof(uint16_t)));
    track(slabdev_, cap_slabs);
    slabdev_.zero();
+   track(ranges_, cap_slabs);
+   ranges_.zero();
    if (mode == CDH_MODE_FP64_FUSED) {
       track(state_, (size_t)N_STATE * S);
       track(A0_, S);
@@ -339,6 +341,10 @@ int Engine::add(const SlabHost &h)
    d.nstim = (int32_t)h.stim.size();
    d.scale = h.diffusionScale;
    d.threshold = h.threshold;
+   d.check_range = h.check_ranges;
+   d.vm_min = h.vm_min;
+   d.vm_max = h.vm_max;
+   put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));
    for (int q = 0; q < d.nstim; q++) {
       const cdh_box_stim &b = h.stim[q];
       d.stim[q] = PulseDev{b.t0, b.tf, b.period, b.phase, b.duration, b.amplitude};
@@ -393,6 +399,7 @@ EngineArgs Engine::args(int parity) const
    a.apd_phase = apd_phase_.p;
    a.vmax = vmax_.p;
    a.t90 = t90_.p;
+   a.ranges = ranges_.p;
    return a;
 }
 
@@ -523,6 +530,7 @@ double Engine::autotune(int steps, std::vector<std::pair<LaunchCfg, double>> *ta
    add(vmax_.p, sizeof(double) * S);
    add(t90_.p, sizeof(double) * S);
    add(slabdev_.p, sizeof(SlabDev) * slabs_.size());
+   add(ranges_.p, sizeof(RangeRec) * slabs_.size());
    std::vector<std::unique_ptr<DevBuf<uint8_t>>> saved;
    for (auto &r : regions) {
       saved.emplace_back(new DevBuf<uint8_t>(r.second));
@@ -658,6 +666,18 @@ void Engine::activation(int id, double *tact0, double *tact, double *trec)
    if (tact0) download(tact0_.p + c0, tact0, s.n);
    if (tact) download(tact_.p + c0, tact, s.n);
    if (trec) download(trec_.p + c0, trec, s.n);
+}
+
+RangeRec Engine::range_report(int id)
+{
+   check_slab(id);
+   sync();
+   RangeRec r{};
+   download(ranges_.p + id, &r, 1);
+   put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));
+   sync();
+   if (r.taken) r.cell -= slabs_[id].tile0 * kBlk;
+   return r;
 }
 
 void Engine::t90(int id, double *out)

@@ -85,6 +85,9 @@ class Engine {
    void set_vm(int id, const std::vector<double> &vm);
    void activation(int id, double *tact0, double *tact, double *trec);   // slab order
    void t90(int id, double *out);
+   // out-of-range record since the last call (cell as a slab-order index);
+   // clears it
+   RangeRec range_report(int id);
 
    int mode() const { return mode_; }
    int64_t device_bytes() const { return bytes_; }
@@ -138,6 +141,7 @@ class Engine {
    DevBuf<double> box_[2];
    DevBuf<double> tact0_, tact_, trec_, vmax_, t90_;
    DevBuf<uint8_t> apd_phase_;
+   DevBuf<RangeRec> ranges_;
 };
 
 // mixed-mode FP32 slots (tt06_mixed.hpp order) and the reference slot each

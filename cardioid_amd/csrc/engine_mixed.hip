@@ -17,8 +17,8 @@ using namespace mixed;
 
 namespace {
 
-__device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S, const SlabView &v,
-                                           int blk, int lane)
+__device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S, int sl,
+                                           const SlabView &v, int blk, int lane)
 {
    const int64_t i = (int64_t)blk * kBlk + lane;
    // padding cells of a slab's last tile compute on a copy of a tissue
@@ -54,6 +54,7 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
       acc = __builtin_fmaf(__builtin_nontemporal_load(wp + kBlk * (k - 1)),
                            (float)(box_load(R, b8, S.soff[k]) - V), acc);
    const double dVD = (double)acc * v.scale;
+   engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);
    // K_i follows dVK_i = K_i/c9 + Vm of the reference: dK_i = c9*dt*(iK - dVm_D - iStim)
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_mixed(EngineArgs a)
    if (!S.active) return;
    SlabView v;
    slab_view(S, a.parity, a.dt, v);
-   cell_mixed(a, S, v, blk, lane);
+   cell_mixed(a, S, sl, v, blk, lane);
    if (blk == S.first_blk && lane == 0) slab_advance(S, a.parity, v);
 }
 

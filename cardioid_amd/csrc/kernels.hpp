@@ -127,6 +127,33 @@ __device__ __forceinline__ void sensor_eval_pair(const SensorArgs &a, int i, dou
    if (a.apd) apd_eval(a, i, v1, vrest, now);
 }
 
+// Voltage range check (Simulate::checkRanges, elec/Simulate.cc:35-53):
+// cells whose Vm at the start of a step lies outside [vmin, vmax].  The
+// reference prints one warning per such cell (Simulate::outOfRange, :55-74);
+// on the device each wavefront adds its count with one atomic and the first
+// wavefront to find one records it.
+struct RangeRec {
+   unsigned long long count;    // out-of-range (cell, step) pairs
+   int32_t taken;               // 1 once the first record is written
+   int32_t loop;                // loop counter of that step (before its increment)
+   int64_t cell;                // engine cell index
+   double V, dVmR, dVmD;
+};
+
+__device__ __forceinline__ void range_report(RangeRec *r, uint64_t mask, int lane, int loop,
+                                             int64_t cell, double V, double dVmR, double dVmD)
+{
+   if (lane != __builtin_ctzll(mask)) return;
+   atomicAdd(&r->count, (unsigned long long)__builtin_popcountll(mask));
+   if (atomicCAS(&r->taken, 0, 1) == 0) {
+      r->loop = loop;
+      r->cell = cell;
+      r->V = V;
+      r->dVmR = dVmR;
+      r->dVmD = dVmD;
+   }
+}
+
 // kernels (kernels_f64.hip)
 template <bool BLK>
 __global__ void k_tt06_calc_f64(ReactionArgs a);
@@ -139,5 +166,7 @@ __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR
 __global__ void k_fill_f64(int n, double v, double *x);
 __global__ void k_set_list_f64(int n, const int *list, double v, double *x);
 __global__ void k_sensor_f64(SensorArgs a, const double *Vm, const double *vrest);
+__global__ void k_check_range_f64(int n, const double *Vm, const double *dVmR, const double *dVmD,
+                                  double vmin, double vmax, int loop, RangeRec *r);
 
 }  // namespace cdh

@@ -133,4 +133,18 @@ __global__ __launch_bounds__(256) void k_sensor_f64(SensorArgs a, const double *
    sensor_eval(a, i, Vm[i], a.apd ? vrest[i] : 0.0);
 }
 
+// Simulate::checkRanges on Vm before the integrator (elec/simulationLoop.cc:214-217).
+__global__ __launch_bounds__(256) void k_check_range_f64(int n, const double *__restrict__ Vm,
+                                                         const double *__restrict__ dVmR,
+                                                         const double *__restrict__ dVmD,
+                                                         double vmin, double vmax, int loop,
+                                                         RangeRec *r)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   const double v = i < n ? Vm[i] : 0.0;
+   const bool out = i < n && (v > vmax || v < vmin);
+   const uint64_t m = __ballot(out);
+   if (m) range_report(r, m, threadIdx.x & 63, loop, i, v, out ? dVmR[i] : 0.0, out ? dVmD[i] : 0.0);
+}
+
 }  // namespace cdh

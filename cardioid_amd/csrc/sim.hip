@@ -64,6 +64,7 @@ struct cdh_sim {
       DevBuf<double> tact0, tact, trec;
       DevBuf<uint8_t> apdPhase;
       DevBuf<double> vmax, t90;
+      DevBuf<RangeRec> range;
       int64_t bytes = 0;
       ~Unfused()
       {
@@ -151,6 +152,9 @@ void step_unfused(cdh_sim *s)
    d.scale = s->p.diffusionScale;
    d.dVm = u->dVmD.p;
    hipLaunchKernelGGL(k_fgr_calc_f64<true>, g, b, 0, u->stream, d);
+   if (s->p.check_ranges)
+      hipLaunchKernelGGL(k_check_range_f64, g, b, 0, u->stream, n, u->Vm.p, u->dVmR.p, u->dVmD.p,
+                         s->p.vm_min, s->p.vm_max, s->loop, u->range.p);
    // integrator and sensors at the time after the step
    s->time += s->p.dt;
    ++s->loop;
@@ -285,6 +289,8 @@ void create_unfused(cdh_sim *s, const SlabHost &h, const std::vector<double> &co
    track(u.get(), u->stimListDev, lists.size() ? lists.size() : 1);
    u->stimListDev.upload(lists.data(), lists.size());
    std::vector<double> neg(S, -1.0);
+   track(u.get(), u->range, 1);
+   u->range.zero();
    track(u.get(), u->tact0, S);
    u->tact0.upload(neg.data(), S);
    track(u.get(), u->tact, S);
@@ -358,6 +364,9 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
          in.gate_tail_rule = p->gate_tail_rule;
          in.nstim = nstim;
          in.stim = stim;
+         in.check_ranges = p->check_ranges;
+         in.vm_min = p->vm_min;
+         in.vm_max = p->vm_max;
          const SlabHost h = prepare_slab(in, s->K.c9);
          if (p->mode == CDH_MODE_FP64_UNFUSED) {
             create_unfused(s.get(), h, coef);
@@ -525,6 +534,29 @@ int cdh_sim_get_t90(cdh_sim *s, double *t90)
       else s->u->t90.download(v.data(), s->n);
       const std::vector<int64_t> &perm = sim_perm(s);
       for (int64_t k = 0; k < s->n; k++) t90[perm[k]] = v[k];
+   });
+}
+
+int cdh_sim_range_report(cdh_sim *s, int64_t *count, int *loop, int64_t *cell, double *V,
+                         double *dVmR, double *dVmD)
+{
+   return guard([&] {
+      CDH_REQUIRE(s, "cdh_sim_range_report: null");
+      RangeRec r{};
+      if (s->eng) {
+         r = s->eng->range_report(s->slab);
+      } else if (s->u) {
+         sim_sync(s);
+         s->u->range.download(&r, 1);
+         s->u->range.zero();
+      }
+      if (count) *count = (int64_t)r.count;
+      if (!r.taken) return;
+      if (loop) *loop = r.loop;
+      if (cell) *cell = sim_perm(s)[r.cell];
+      if (V) *V = r.V;
+      if (dVmR) *dVmR = r.dVmR;
+      if (dVmD) *dVmD = r.dVmD;
    });
 }
 

@@ -54,6 +54,9 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
    h.S = h.ntile * kBlk;
    h.diffusionScale = in.diffusionScale;
    h.threshold = in.threshold;
+   h.check_ranges = in.check_ranges;
+   h.vm_min = in.vm_min;
+   h.vm_max = in.vm_max;
    const int64_t S = h.S;
 
    // box and slab order

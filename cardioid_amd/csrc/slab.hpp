@@ -31,6 +31,8 @@ struct SlabInput {
    int gate_tail_rule = 1;
    int nstim = 0;
    const cdh_box_stim *stim = nullptr;
+   int check_ranges = 1;               // Simulate::checkRanges with [vm_min, vm_max]
+   double vm_min = -150.0, vm_max = 100.0;
 };
 
 struct SlabHost {
@@ -51,6 +53,8 @@ struct SlabHost {
    std::vector<cdh_box_stim> stim;
    std::vector<std::vector<int>> stimList;   // slab-order cells per stimulus
    double diffusionScale = 0, threshold = -40.0;
+   int check_ranges = 1;
+   double vm_min = -150.0, vm_max = 100.0;
 };
 
 // Validates the input and builds the slab (elec/FGRDiffusionOMP.cc:14-73

@@ -63,7 +63,8 @@ class SimParams(C.Structure):
                 ("celltype", u8p), ("sigma", dp), ("ncelltype", C.c_int),
                 ("celltypes", C.POINTER(CellType)), ("scales", dp), ("fit_l", ip), ("fit_m", ip),
                 ("fit_coef", dp), ("threshold", C.c_double), ("mode", C.c_int),
-                ("gate_tail_rule", C.c_int), ("sensors", C.c_int), ("variant", C.c_int)]
+                ("gate_tail_rule", C.c_int), ("sensors", C.c_int), ("variant", C.c_int),
+                ("check_ranges", C.c_int), ("vm_min", C.c_double), ("vm_max", C.c_double)]
 
 
 class BatchParams(C.Structure):
@@ -78,7 +79,8 @@ class SlabDesc(C.Structure):
                 ("ncell", C.c_int64), ("gid", i64p), ("celltype", u8p), ("sigma", dp),
                 ("ncelltype", C.c_int), ("celltypes", C.POINTER(CellType)), ("scales", dp),
                 ("threshold", C.c_double), ("gate_tail_rule", C.c_int), ("nstim", C.c_int),
-                ("stim", C.POINTER(BoxStim))]
+                ("stim", C.POINTER(BoxStim)), ("check_ranges", C.c_int), ("vm_min", C.c_double),
+                ("vm_max", C.c_double)]
 
 
 EXPORTS = [
@@ -94,13 +96,13 @@ EXPORTS = [
     "cdh_activation_sensor_destroy",
     "cdh_sim_create", "cdh_sim_destroy", "cdh_sim_run", "cdh_sim_sync", "cdh_sim_run_timed",
     "cdh_sim_get_vm", "cdh_sim_get_state", "cdh_sim_set_state", "cdh_sim_get_activation",
-    "cdh_sim_get_t90",
+    "cdh_sim_get_t90", "cdh_sim_range_report",
     "cdh_sim_time", "cdh_sim_loop", "cdh_sim_info",
     "cdh_batch_create", "cdh_batch_destroy", "cdh_slab_extent", "cdh_batch_add",
     "cdh_batch_remove", "cdh_batch_run", "cdh_batch_run_timed", "cdh_batch_sync",
     "cdh_batch_set_graph", "cdh_batch_set_launch", "cdh_batch_get_launch", "cdh_batch_autotune",
     "cdh_batch_get_vm", "cdh_batch_get_state", "cdh_batch_get_activation", "cdh_batch_get_t90",
-    "cdh_batch_slab_info", "cdh_batch_info",
+    "cdh_batch_range_report", "cdh_batch_slab_info", "cdh_batch_info",
 ]
 
 ERR_FULL = 5
@@ -169,6 +171,8 @@ def lib():
         L.cdh_sim_get_activation.argtypes = [C.c_void_p, dp, dp, dp]
         L.cdh_sim_get_t90.argtypes = [C.c_void_p, dp]
         L.cdh_sim_info.argtypes = [C.c_void_p, i64p, ip, ip, ip]
+        L.cdh_sim_range_report.argtypes = [C.c_void_p, i64p, ip, i64p, dp, dp, dp]
+        L.cdh_batch_range_report.argtypes = [C.c_void_p, C.c_int, i64p, ip, i64p, dp, dp, dp]
         L.cdh_celltype_standard.argtypes = [C.c_char_p, C.POINTER(CellType)]
         L.cdh_batch_create.argtypes = [C.POINTER(BatchParams), C.POINTER(C.c_void_p)]
         L.cdh_batch_destroy.argtypes = [C.c_void_p]
@@ -202,6 +206,16 @@ def ptr(a, t=dp):
     return a.ctypes.data_as(t)
 
 
+def _range_report(call, what):
+    n, loop, cell = C.c_int64(), C.c_int(), C.c_int64()
+    v, r, d = C.c_double(), C.c_double(), C.c_double()
+    check(call(C.byref(n), C.byref(loop), C.byref(cell), C.byref(v), C.byref(r), C.byref(d)), what)
+    if n.value == 0:
+        return None
+    return dict(count=n.value, loop=loop.value, cell=cell.value, V=v.value, dVmR=r.value,
+                dVmD=d.value)
+
+
 def standard_celltype(name: str) -> CellType:
     ct = CellType()
     check(lib().cdh_celltype_standard(name.encode(), C.byref(ct)), "cdh_celltype_standard")
@@ -218,7 +232,8 @@ class Simulation:
     def __init__(self, anatomy, celltypes: Sequence[str], dt: float, diffusion_scale: float,
                  stimuli=(), mode: str = "fp64_fused", threshold: float = -40.0,
                  fits: Optional[FitSet] = None, scales: Optional[np.ndarray] = None,
-                 gate_tail_rule: bool = True, sensors: int = 1, variant: int = 0):
+                 gate_tail_rule: bool = True, sensors: int = 1, variant: int = 0,
+                 check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0):
         L = lib()
         self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
         fl, fm, fc = self.fits.packed()
@@ -241,7 +256,8 @@ class Simulation:
                       scales=ptr(sc) if sc is not None else None,
                       fit_l=ptr(fl, ip), fit_m=ptr(fm, ip), fit_coef=ptr(fc), threshold=threshold,
                       mode=MODES[mode], gate_tail_rule=int(gate_tail_rule), sensors=int(sensors),
-                      variant=variant)
+                      variant=variant, check_ranges=int(check_ranges), vm_min=vm_min,
+                      vm_max=vm_max)
         st = (BoxStim * max(1, len(stimuli)))(*stimuli)
         h = C.c_void_p()
         check(L.cdh_sim_create(C.byref(p), len(stimuli), st, C.byref(h)), "cdh_sim_create")
@@ -297,6 +313,13 @@ class Simulation:
         check(lib().cdh_sim_get_t90(self.h, ptr(out)), "cdh_sim_get_t90")
         return out
 
+    def range_report(self):
+        """Out-of-range Vm since the last call: None, or a dict with the
+        count of (cell, step) pairs and the values Simulate::outOfRange
+        prints for one of them."""
+        return _range_report(lambda *a: lib().cdh_sim_range_report(self.h, *a),
+                             "cdh_sim_range_report")
+
     @property
     def time(self) -> float:
         return lib().cdh_sim_time(self.h)
@@ -318,7 +341,8 @@ class Slab:
     anatomy with its cell types, stimuli and per-cell drug scales."""
 
     def __init__(self, anatomy, celltypes: Sequence[str], diffusion_scale: float, stimuli=(),
-                 scales=None, threshold: float = -40.0, gate_tail_rule: bool = True):
+                 scales=None, threshold: float = -40.0, gate_tail_rule: bool = True,
+                 check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0):
         cts = (CellType * len(celltypes))(*[standard_celltype(n) for n in celltypes])
         self.ncell = int(anatomy.gid.size)
         gid = np.ascontiguousarray(anatomy.gid, dtype=np.int64)
@@ -337,7 +361,8 @@ class Slab:
                              ncell=self.ncell, gid=ptr(gid, i64p), celltype=ptr(ctype, u8p),
                              sigma=ptr(sigma), ncelltype=len(celltypes), celltypes=cts,
                              scales=ptr(sc) if sc is not None else None, threshold=threshold,
-                             gate_tail_rule=int(gate_tail_rule), nstim=len(stimuli), stim=st)
+                             gate_tail_rule=int(gate_tail_rule), nstim=len(stimuli), stim=st,
+                             check_ranges=int(check_ranges), vm_min=vm_min, vm_max=vm_max)
 
     def extent(self):
         """(cells, box points, parameter sets) the slab takes in an arena."""
@@ -447,6 +472,10 @@ class Batch:
         out = np.zeros(self.ncell[sid])
         check(lib().cdh_batch_get_t90(self.h, sid, ptr(out)), "cdh_batch_get_t90")
         return out
+
+    def range_report(self, sid: int):
+        return _range_report(lambda *a: lib().cdh_batch_range_report(self.h, sid, *a),
+                             "cdh_batch_range_report")
 
     def slab_info(self, sid: int):
         n, t, l = C.c_int64(), C.c_double(), C.c_int()

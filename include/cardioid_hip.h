@@ -196,7 +196,10 @@ typedef struct {
    int gate_tail_rule;          /* 1: the v2 loops' per-group bounds (reference) */
    int sensors;                 /* bit 0: activation times; bit 1: APD90 as well */
    int variant;                 /* fused kernel occupancy target, waves/SIMD (FP64: 2..4,
-                                   mixed: 3..6); 0 = default */
+                                   mixed: 3..5); 0 = default */
+   int check_ranges;            /* Simulate::checkRanges (checkRanges, default on) */
+   double vm_min, vm_max;       /* its range (VmMin, VmMax: -150, 100 by default,
+                                   initializeSimulate.cc:96-98) */
 } cdh_sim_params;
 
 int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim, cdh_sim **out);
@@ -216,6 +219,14 @@ int cdh_sim_get_activation(cdh_sim *s, double *tact0, double *tact, double *trec
  * threshold, t90 is the first time Vm < Vrest + 0.1*(Vmax - Vrest), with
  * Vrest the cell type's rest Vm and Vmax the largest Vm since activation. */
 int cdh_sim_get_t90(cdh_sim *s, double *t90);
+/* Vm range check (Simulate::checkRanges, Simulate.cc:35-53, before the
+ * integrator): the number of (cell, step) pairs whose Vm left [vm_min,
+ * vm_max] since the last call, and one of them (the first step's) with the
+ * values Simulate::outOfRange prints (loop, cell in input order, V, dVmR,
+ * dVmD).  count 0 leaves the other outputs untouched; the call clears the
+ * record. */
+int cdh_sim_range_report(cdh_sim *s, int64_t *count, int *loop, int64_t *cell, double *V,
+                         double *dVmR, double *dVmD);
 double cdh_sim_time(const cdh_sim *s);
 int cdh_sim_loop(const cdh_sim *s);
 /* bytes of device memory held, and the box dimensions used */
@@ -261,6 +272,8 @@ typedef struct {
    int gate_tail_rule;
    int nstim;                   /* <= 8 */
    const cdh_box_stim *stim;
+   int check_ranges;            /* as cdh_sim_params */
+   double vm_min, vm_max;
 } cdh_slab_desc;
 
 int cdh_batch_create(const cdh_batch_params *p, cdh_batch **out);
@@ -290,6 +303,8 @@ int cdh_batch_get_vm(cdh_batch *b, int slab, double *vm);
 int cdh_batch_get_state(cdh_batch *b, int slab, double *state);      /* [19][ncell] */
 int cdh_batch_get_activation(cdh_batch *b, int slab, double *tact0, double *tact, double *trec);
 int cdh_batch_get_t90(cdh_batch *b, int slab, double *t90);
+int cdh_batch_range_report(cdh_batch *b, int slab, int64_t *count, int *loop, int64_t *cell,
+                           double *V, double *dVmR, double *dVmD);
 int cdh_batch_slab_info(cdh_batch *b, int slab, int64_t *ncell, double *time, int *loop);
 int cdh_batch_info(cdh_batch *b, int64_t *device_bytes, int64_t *cells_used, int64_t *box_used,
                    int *nslab);

@@ -223,3 +223,31 @@ def test_engine_apd90_matches_oracle():
     assert (t90 > 0).mean() > 0.99
     assert np.array_equal(act, want_act)
     assert np.array_equal(t90, want_t90)
+
+
+@pytest.mark.parametrize("mode", ["fp64_unfused", "fp64_fused", "mixed_fused"])
+def test_range_check_counts_every_out_of_range_cell_step(mode):
+    """Simulate::checkRanges (Vm at the start of each step against [VmMin,
+    VmMax]): the device count equals a host count over Vm read back before
+    every step, and the recorded example is from the first such step."""
+    say(f"range check, {mode}: brick 10x8x6, VmMax -60 mV, 250 steps")
+    a = W.brick(10, 8, 6)
+    vmax = -60.0
+    sim = hip.Simulation(a, ["endoCellML"], DT, W.DIFFUSION_SCALE,
+                         [W.box_stimulus(xMax=3, nx=10, ny=8, nz=6)], mode=mode, vm_max=vmax)
+    host, first, snaps = 0, None, []
+    for step in range(250):
+        v = sim.vm()
+        snaps.append(v)
+        k = int(((v > vmax) | (v < -150.0)).sum())
+        host += k
+        if first is None and k:
+            first = step
+        sim.run(1)
+    rep = sim.range_report()
+    sim.close()
+    say(f"  device count {rep['count']}, host count {host}, first loop {rep['loop']} ({first}), "
+        f"cell {rep['cell']} V {rep['V']:.3f}")
+    assert rep["count"] == host and host > 0
+    assert rep["loop"] == first
+    assert rep["V"] == snaps[first][rep["cell"]] and rep["V"] > vmax

@@ -1,0 +1,120 @@
+"""Serving loop logic on the CPU, with a stand-in engine that counts steps.
+
+Checks: arrival-order admission under an arena limit, every request run for
+exactly its own number of steps, results collected before its room is
+reused, latency = completion - arrival, and an oversized request rejected.
+"""
+import numpy as np
+import pytest
+
+from cardioid_amd import hip, serve, workloads as W
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(dt, 1e-6)
+
+
+class FakeBatch:
+    """Arena of `cap` cells; each run(k) costs k * (active cells) * 1e-9 s."""
+
+    def __init__(self, cap, clock):
+        self.cap, self.clock = cap, clock
+        self.slabs = {}
+        self.next = 0
+        self.log = []
+
+    def add(self, slab):
+        used = sum(s["n"] for s in self.slabs.values())
+        if used + slab.ncell > self.cap:
+            raise hip.ArenaFull("full")
+        sid = self.next
+        self.next += 1
+        self.slabs[sid] = {"n": slab.ncell, "steps": 0}
+        self.log.append(("add", sid, slab.ncell))
+        return sid
+
+    def remove(self, sid):
+        self.log.append(("remove", sid, self.slabs[sid]["steps"]))
+        del self.slabs[sid]
+
+    def run(self, k):
+        for s in self.slabs.values():
+            s["steps"] += k
+        self.clock.t += k * sum(s["n"] for s in self.slabs.values()) * 1e-9
+
+    def sync(self):
+        pass
+
+    def activation(self, sid):
+        n = self.slabs[sid]["n"]
+        v = np.full(n, float(self.slabs[sid]["steps"]))
+        return v, v, v
+
+    def t90(self, sid):
+        return np.full(self.slabs[sid]["n"], 2.0 * self.slabs[sid]["steps"])
+
+
+class FakeSlab:
+    def __init__(self, n):
+        self.ncell = n
+
+
+def fake_make_slab(req):
+    n = int(np.prod(req.dims))
+
+    class A:
+        ncell = n
+    return A(), FakeSlab(n)
+
+
+def test_stream_admission_exact_steps_and_latency():
+    reqs = W.request_stream(30, rate_hz=20.0, seed=11)
+    clock = FakeClock()
+    cap = 3 * 128 ** 3
+    b = FakeBatch(cap, clock)
+    st = serve.Server(b, dt=0.02, chunk=50, make_slab=fake_make_slab).serve(
+        reqs, clock=clock, sleep=clock.sleep)
+    assert len(st.tickets) == 30
+    by_rid = {t.rid: t for t in st.tickets}
+    order = sorted(range(30), key=lambda k: reqs[k].arrival_s)
+    for k, r in enumerate(sorted(reqs, key=lambda r: r.arrival_s)):
+        t = by_rid[k]
+        want = int(round(r.duration_ms / 0.02))
+        assert t.nsteps == want
+        # the engine ran the slab for exactly its steps: activation (fake) = steps
+        assert (t.tact == want).all() and (t.apd90 == want).all()
+        assert t.done >= t.admitted >= t.arrival and t.latency > 0
+    # admissions in arrival order, never above the arena
+    adds = [e for e in b.log if e[0] == "add"]
+    assert [e[2] for e in adds] == [int(np.prod(by_rid[k].request.dims)) for k in range(30)]
+    live = 0
+    for e in b.log:
+        live += e[2] if e[0] == "add" else 0
+        if e[0] == "remove":
+            live -= [a[2] for a in adds if a[1] == e[1]][0]
+        assert live <= cap
+    s = st.summary()
+    assert s["p99_latency_s"] >= s["p50_latency_s"] > 0 and s["max_active"] >= 2
+    assert s["cell_steps"] == sum(t.ncell * t.nsteps for t in st.tickets)
+
+
+def test_request_that_never_fits_is_rejected():
+    reqs = [W.Request(0.0, (64, 64, 64), 10.0, np.ones(3), 0.0)]
+    clock = FakeClock()
+    b = FakeBatch(1000, clock)
+    with pytest.raises(RuntimeError, match="never fit"):
+        serve.Server(b, make_slab=fake_make_slab).serve(reqs, clock=clock, sleep=clock.sleep)
+
+
+def test_arena_for_budget():
+    cells, box = serve.arena_for_budget(16 * 2 ** 30)
+    assert cells % 64 == 0
+    per = cells * serve.MIXED_BYTES_PER_CELL + box * serve.BYTES_PER_BOX_POINT
+    assert per <= 16 * 2 ** 30 and per > 0.99 * 16 * 2 ** 30 - 64 * 300
