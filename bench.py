@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
 """Throughput of Cardioid's electrics step on one MI355X.
 
-Workload (default): 256x256x512 grid (33.5M points) with a truncated
+Headline workload: 256x256x512 grid (33.5M points) with a truncated
 ellipsoidal-shell anatomy at ~60% tissue (endo/mid/epi by transmural depth),
 fibre conductivity with the helix angle rotating -60..+60 deg epi->endo,
 8 seeded endocardial stimuli, dt = 0.02 ms, plain FP64 (the drop-in
 arithmetic), one fused launch per step.  A "step" is one omp-loop step over
-every tissue cell (stimulus, TT06Dev reaction, FGR stencil, Vm update,
-activation sensors).
+every tissue cell (stimulus, TT06Dev reaction, FGR stencil, range check, Vm
+update, activation sensors).
 
-Prints one JSON line (rank 0).  value = tissue cell-steps per second of the
-fused FP64 engine (Gcell-steps/s), measured with HIP events on the engine's
-stream over exactly --steps steps after --warmup untimed ones.
+Beside it, on the same anatomy: the mixed-precision fused step and the
+reference-structured unfused FP64 step; then 64 batched slabs (mixed, graph
+replayed, autotuned launch) and a served request stream (p50/p99 latency).
+
+Prints one JSON line.  value = tissue cell-steps per second of the fused
+FP64 engine (Gcell-steps/s), timed with HIP events on the engine's stream
+over exactly --steps steps after --warmup untimed ones.
 """
 from __future__ import annotations
 
@@ -33,6 +37,7 @@ FUSED_F64_BYTES = 400
 UNFUSED_F64_BYTES = 480
 # mixed: 17 FP32 states r+w (136), Ca_i/Na_i FP64 r+w (32), Vm r+w (16), 18 FP32 weights (72)
 MIXED_BYTES = 256
+DT = 0.02
 
 
 def log(msg):
@@ -49,7 +54,7 @@ def cpu_baseline(budget_s: float = 12.0):
     threads = min(16, os.cpu_count() or 1)
     a, depth = W.ellipsoid_shell(96, 96, 192, seed=0)
     stims = W.endocardial_point_stimuli(a, depth, 8, seed=0)
-    sim = OL.OracleSim(a, W.CELLTYPES_EMI, 0.02, W.DIFFUSION_SCALE, stims, nthreads=threads)
+    sim = OL.OracleSim(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims, nthreads=threads)
     sim.run(2)
     steps = 0
     t0 = time.perf_counter()
@@ -75,12 +80,77 @@ def cpu_baseline(budget_s: float = 12.0):
                       f"{el:.1f} s, oracle/liboracle.so with {threads} OpenMP threads on {cpu}"}
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "step_fused_f64_shell256x256x512_traffic.json")
+def load_traffic(stem):
+    path = os.path.join(ROOT, "profiles", stem + "_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         return json.load(fh).get("bytes_per_launch")
+
+
+def fused_run(hip, slab, ncell, mode, steps, warmup, tune_steps, graph):
+    """One slab in the engine: autotuned launch, graph replay, HIP-event time."""
+    cells, box, _ = slab.extent()
+    b = hip.Batch(DT, cells, box, slab_capacity=1, mode=mode)
+    sid = b.add(slab)
+    b.sync()
+    best, table = b.autotune(tune_steps) if tune_steps > 0 else (None, [])
+    b.set_graph(graph)
+    b.run(warmup)
+    b.sync()
+    ms = b.run_timed(steps)
+    vm = b.vm(sid)
+    act = b.activation(sid)[1]
+    rng = b.range_report(sid)
+    launch = b.launch()
+    b.close()
+    return dict(ms=ms, steps=steps, rate=ncell * steps / (ms * 1e-3) / 1e9, vm=vm, act=act,
+                launch=launch, tune=table, range=rng)
+
+
+def batched(hip, W, steps, tune_steps, graph):
+    cases = W.batched_slabs()
+    slabs = [hip.Slab(c.anatomy, ["endoCellML"], W.DIFFUSION_SCALE, c.stimuli, scales=c.scales)
+             for c in cases]
+    ext = [s.extent() for s in slabs]
+    ncell = sum(s.ncell for s in slabs)
+    b = hip.Batch(DT, sum(e[0] for e in ext), sum(e[1] for e in ext), slab_capacity=len(slabs),
+                  mode="mixed_fused")
+    t = time.perf_counter()
+    ids = [b.add(s) for s in slabs]
+    b.sync()
+    log(f"batched: {len(slabs)} slabs, {ncell} cells, added in {time.perf_counter() - t:.1f} s")
+    best, table = b.autotune(tune_steps)
+    b.set_graph(graph)
+    b.run(10)
+    b.sync()
+    ms = b.run_timed(steps)
+    act = [b.activation(i)[1] for i in ids]
+    launch = b.launch()
+    b.close()
+    return {"value": ncell * steps / (ms * 1e-3) / 1e9, "unit": "Gcell-steps/s",
+            "slabs": len(slabs), "tissue_cells": ncell, "steps": steps,
+            "ms_per_step": ms / steps,
+            "launch": {"waves": launch[0], "block": launch[1], "occupancy": launch[2]},
+            "graph_steps": graph,
+            "slabs_activated": int(sum((a >= 0).mean() > 0.5 for a in act)),
+            "achieved_GBps": MIXED_BYTES * ncell * steps / (ms * 1e-3) / 1e9}
+
+
+def serving(hip, W, nreq, rate, hbm_gb, graph):
+    from cardioid_amd import serve
+    reqs = W.request_stream(nreq, rate_hz=rate, seed=4)
+    cells, box = serve.arena_for_budget(int(hbm_gb * 2 ** 30))
+    b = hip.Batch(DT, cells, box, slab_capacity=1024, mode="mixed_fused", sensors=3)
+    b.set_launch(3, 64)
+    b.set_graph(graph)
+    st = serve.Server(b, chunk=100).serve(reqs)
+    b.close()
+    s = st.summary()
+    s.update({"rate_hz": rate, "hbm_budget_gb": hbm_gb, "arena_cells": cells,
+              "request_dims": "each axis log-uniform 32..128, duration 200-1000 ms, "
+                              "scales U[0,1], mixed precision"})
+    return s
 
 
 def main():
@@ -91,6 +161,12 @@ def main():
     ap.add_argument("--grid", type=str, default="256,256,512")
     ap.add_argument("--unfused-steps", type=int, default=500)
     ap.add_argument("--mixed-steps", type=int, default=5000)
+    ap.add_argument("--tune-steps", type=int, default=20)
+    ap.add_argument("--graph", type=int, default=10)
+    ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -98,6 +174,7 @@ def main():
               "path is not sharded across GPUs here (--gpus must be 1).", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     from cardioid_amd import hip, workloads as W
     nx, ny, nz = (int(v) for v in args.grid.split(","))
     log(f"generating {nx}x{ny}x{nz} ellipsoidal shell")
@@ -106,32 +183,32 @@ def main():
     stims = W.endocardial_point_stimuli(anat, depth, 8, seed=0)
     log(f"{anat.ncell} tissue cells ({anat.ncell / (nx * ny * nz):.3f} of the grid) "
         f"in {time.perf_counter() - t:.1f} s")
+    slab = hip.Slab(anat, W.CELLTYPES_EMI, W.DIFFUSION_SCALE, stims)
 
-    results = {}
-    for mode, steps in (("fp64_fused", args.steps), ("mixed_fused", args.mixed_steps),
-                        ("fp64_unfused", args.unfused_steps)):
-        if steps <= 0:
-            continue
+    res = {}
+    for mode, steps in (("fp64_fused", args.steps), ("mixed_fused", args.mixed_steps)):
         t = time.perf_counter()
-        sim = hip.Simulation(anat, W.CELLTYPES_EMI, 0.02, W.DIFFUSION_SCALE, stims, mode=mode)
-        log(f"{mode}: engine ready in {time.perf_counter() - t:.1f} s, {sim.info()}")
+        r = fused_run(hip, slab, anat.ncell, mode, steps, args.warmup, args.tune_steps, args.graph)
+        res[mode] = r
+        log(f"{mode}: launch {r['launch']}, {steps} steps in {r['ms']:.1f} ms -> "
+            f"{r['rate']:.3f} Gcell-steps/s ({time.perf_counter() - t:.1f} s incl. set-up)")
+    if args.unfused_steps > 0:
+        sim = hip.Simulation(anat, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims,
+                             mode="fp64_unfused")
         sim.run(args.warmup)
         sim.sync()
-        ms = sim.run_timed(steps)
-        vm = sim.vm()
-        act = sim.activation()[1]
+        ms = sim.run_timed(args.unfused_steps)
         sim.close()
-        rate = anat.ncell * steps / (ms * 1e-3) / 1e9
-        log(f"{mode}: {steps} steps in {ms:.1f} ms -> {rate:.3f} Gcell-steps/s; "
-            f"Vm range [{vm.min():.2f}, {vm.max():.2f}], activated {(act >= 0).mean():.3f}")
-        results[mode] = dict(ms=ms, steps=steps, rate=rate)
+        res["fp64_unfused"] = dict(ms=ms, steps=args.unfused_steps,
+                                   rate=anat.ncell * args.unfused_steps / (ms * 1e-3) / 1e9)
+        log(f"fp64_unfused: {res['fp64_unfused']['rate']:.3f} Gcell-steps/s")
 
-    fused = results["fp64_fused"]
-    ms_per_step = fused["ms"] / fused["steps"]
+    f64, mx = res["fp64_fused"], res["mixed_fused"]
+    ms_per_step = f64["ms"] / f64["steps"]
     achieved = FUSED_F64_BYTES * anat.ncell / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
-        "value": fused["rate"],
+        "value": f64["rate"],
         "unit": "Gcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -145,24 +222,47 @@ def main():
         "config": {"workload": f"ellipsoidal shell {nx}x{ny}x{nz}, ~60% tissue, endo/mid/epi, "
                                "rotating fibres, 8 endocardial stimuli, dt 0.02 ms, plain FP64, "
                                "fused step",
-                   "grid_points": nx * ny * nz, "tissue_cells": anat.ncell, "dt_ms": 0.02,
-                   "mode": "fp64_fused"},
+                   "grid_points": nx * ny * nz, "tissue_cells": anat.ncell, "dt_ms": DT,
+                   "mode": "fp64_fused",
+                   "launch": {"waves": f64["launch"][0], "block": f64["launch"][1],
+                              "occupancy": f64["launch"][2]},
+                   "graph_steps": args.graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
-                     "kernel": "k_step_fused_f64", "algorithmic_bytes_per_cell_step": FUSED_F64_BYTES},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic("engine_f64_shell256x256x512"),
+                     "kernel": "k_engine_f64", "algorithmic_bytes_per_cell_step": FUSED_F64_BYTES},
     }
-    if "mixed_fused" in results:
-        mx = results["mixed_fused"]
-        mms = mx["ms"] / mx["steps"]
-        out["mixed_fused"] = {"value": mx["rate"], "unit": "Gcell-steps/s", "steps": mx["steps"],
-                              "ms_per_step": mms, "speedup_vs_fp64_fused": mx["rate"] / fused["rate"],
-                              "algorithmic_bytes_per_cell_step": MIXED_BYTES,
-                              "achieved_GBps": MIXED_BYTES * anat.ncell / (mms * 1e-3) / 1e9}
-    if "fp64_unfused" in results:
-        u = results["fp64_unfused"]
+    mms = mx["ms"] / mx["steps"]
+    both = (f64["act"] >= 0) & (mx["act"] >= 0)
+    out["mixed_fused"] = {
+        "value": mx["rate"], "unit": "Gcell-steps/s", "steps": mx["steps"], "ms_per_step": mms,
+        "speedup_vs_fp64_fused": mx["rate"] / f64["rate"],
+        "launch": {"waves": mx["launch"][0], "block": mx["launch"][1],
+                   "occupancy": mx["launch"][2]},
+        "algorithmic_bytes_per_cell_step": MIXED_BYTES,
+        "achieved_GBps": MIXED_BYTES * anat.ncell / (mms * 1e-3) / 1e9,
+        "traffic": load_traffic("engine_mixed_shell256x256x512")}
+    if f64["steps"] == mx["steps"]:
+        out["mixed_fused"]["error_vs_fp64_at_end"] = {
+            "t_ms": (args.warmup + f64["steps"]) * DT,
+            "linf_vm_mV": float(np.abs(mx["vm"] - f64["vm"]).max()),
+            "activation_max_abs_ms": float(np.abs(mx["act"] - f64["act"])[both].max()),
+            "activated_same_cells": bool(np.array_equal(f64["act"] >= 0, mx["act"] >= 0))}
+    out["range_check"] = {"fp64_out_of_range": (f64["range"] or {}).get("count", 0),
+                          "mixed_out_of_range": (mx["range"] or {}).get("count", 0)}
+    if "fp64_unfused" in res:
+        u = res["fp64_unfused"]
         out["fp64_unfused"] = {"value": u["rate"], "unit": "Gcell-steps/s", "steps": u["steps"],
                                "ms_per_step": u["ms"] / u["steps"],
                                "algorithmic_bytes_per_cell_step": UNFUSED_F64_BYTES}
+    if args.batched_steps > 0:
+        out["batched"] = batched(hip, W, args.batched_steps, args.tune_steps, args.graph)
+        log(f"batched: {out['batched']['value']:.3f} Gcell-steps/s")
+    if args.serve_requests > 0:
+        out["serving"] = serving(hip, W, args.serve_requests, args.serve_rate, args.serve_hbm_gb,
+                                 args.graph)
+        log(f"serving: p50 {out['serving']['p50_latency_s']:.2f} s, "
+            f"p99 {out['serving']['p99_latency_s']:.2f} s")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores)")
         out["cpu_baseline"] = cpu_baseline()

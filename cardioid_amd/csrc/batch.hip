@@ -160,17 +160,18 @@ int cdh_batch_set_graph(cdh_batch *b, int steps)
    return guard([&] { engine(b).set_graph_steps(steps); });
 }
 
-int cdh_batch_set_launch(cdh_batch *b, int waves, int block)
+int cdh_batch_set_launch(cdh_batch *b, int waves, int block, int occupancy)
 {
-   return guard([&] { engine(b).set_launch(LaunchCfg{waves, block}); });
+   return guard([&] { engine(b).set_launch(LaunchCfg{waves, block, occupancy}); });
 }
 
-int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block)
+int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block, int *occupancy)
 {
    return guard([&] {
       const LaunchCfg c = engine(b).launch();
       if (waves) *waves = c.waves;
       if (block) *block = c.block;
+      if (occupancy) *occupancy = c.occ;
    });
 }
 
@@ -183,9 +184,10 @@ int cdh_batch_autotune(cdh_batch *b, int steps, double *best_ms, double *table, 
       if (best_ms) *best_ms = best;
       int k = 0;
       for (; k < (int)t.size() && k < max_rows && table; k++) {
-         table[3 * k] = t[k].first.waves;
-         table[3 * k + 1] = t[k].first.block;
-         table[3 * k + 2] = t[k].second;
+         table[4 * k] = t[k].first.waves;
+         table[4 * k + 1] = t[k].first.block;
+         table[4 * k + 2] = t[k].first.occ;
+         table[4 * k + 3] = t[k].second;
       }
       if (nrows) *nrows = (int)t.size();
    });

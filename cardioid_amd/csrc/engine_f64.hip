@@ -27,8 +27,18 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    const __amdgpu_buffer_rsrc_t R = box_rsrc(a.box_cur);
    const double V = box_load(R, b8, S.soff[0]);
 
-   // states and weights stream once per step (non-temporal); the tile's
-   // base address is wave-uniform
+   // the stencil first (FGRDiffusionOMP::calc order: A0*phi0, then
+   // A_k*phi_k for k = 1..18): its neighbour and weight loads are issued with
+   // the state loads at the start, so the reaction's arithmetic covers them
+   const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
+   double tmp = __builtin_nontemporal_load(a.A0 + i) * V;
+#pragma unroll
+   for (int k = 1; k < 19; k++)
+      tmp += (double)__builtin_nontemporal_load(wp + kBlk * (k - 1)) * box_load(R, b8, S.soff[k]);
+   const double dVD = tmp * v.scale;
+
+   // states stream once per step (non-temporal); the tile's base address is
+   // wave-uniform
    double *sp = a.state + (int64_t)blk * (N_STATE * kBlk) + lane;
    double st[N_STATE];
 #pragma unroll
@@ -39,12 +49,6 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 #pragma unroll
    for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
 
-   const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
-   double tmp = __builtin_nontemporal_load(a.A0 + i) * V;
-#pragma unroll
-   for (int k = 1; k < 19; k++)
-      tmp += (double)__builtin_nontemporal_load(wp + kBlk * (k - 1)) * box_load(R, b8, S.soff[k]);
-   const double dVD = tmp * v.scale;
    engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);

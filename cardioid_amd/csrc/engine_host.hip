@@ -122,9 +122,20 @@ std::vector<LaunchCfg> Engine::candidates(int mode)
 {
    std::vector<LaunchCfg> c;
    const int w0 = mode == CDH_MODE_MIXED_FUSED ? 3 : 2;
-   for (int w = w0; w <= w0 + 2; w++)
-      for (int b : {64, 128, 256}) c.push_back(LaunchCfg{w, b});
+   for (int w = w0; w <= w0 + 1; w++)
+      for (int b : {64, 128, 256})
+         for (int o : {0, 2, 3}) c.push_back(LaunchCfg{w, b, o});
    return c;
+}
+
+// LDS a workgroup reserves so that at most occ waves per SIMD (4*occ per
+// CU) are resident: just under 1/(workgroups per CU) of the CU's LDS
+int Engine::lds_bytes(const LaunchCfg &c) const
+{
+   if (c.occ <= 0) return 0;
+   const int wpg = c.block / kBlk;
+   const int nwg = std::max(1, 4 * c.occ / wpg);
+   return std::max(0, (lds_per_cu_ / nwg - 256) / 512 * 512);
 }
 
 // ---------------------------------------------------------------- Engine
@@ -175,6 +186,10 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    params_ = RangeAlloc(cap_params);
    slabs_.resize(cap_slabs);
 
+   int dev = 0;
+   CDH_HIP(hipGetDevice(&dev));
+   CDH_HIP(hipDeviceGetAttribute(&lds_per_cu_, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                 dev));
    CDH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
    CDH_HIP(hipEventCreate(&ev0_));
    CDH_HIP(hipEventCreate(&ev1_));
@@ -408,7 +423,7 @@ void Engine::launch_step(int parity)
    const EngineArgs a = args(parity);
    if (a.ntile == 0) return;
    const dim3 grid(8 * a.groups_per_xcd), blk(cfg_.block);
-   hipLaunchKernelGGL(kernel_for(mode_, cfg_.waves), grid, blk, 0, stream_, a);
+   hipLaunchKernelGGL(kernel_for(mode_, cfg_.waves), grid, blk, lds_bytes(cfg_), stream_, a);
    CDH_LAUNCH_CHECK();
 }
 
@@ -463,6 +478,7 @@ void Engine::set_launch(const LaunchCfg &c)
    CDH_REQUIRE(c.waves >= w0 && c.waves <= w0 + 2, "engine: no kernel for that occupancy");
    CDH_REQUIRE(c.block == 64 || c.block == 128 || c.block == 256,
                "engine: workgroup size must be 64, 128 or 256");
+   CDH_REQUIRE(c.occ >= 0 && c.occ <= 8, "engine: occupancy cap must be 0..8");
    cfg_ = c;
    drop_graph();
 }
@@ -474,7 +490,7 @@ void Engine::run(int nsteps)
    while (nsteps > 0) {
       if (graph_steps_ > 0 && parity_ == 0 && nsteps >= graph_steps_) {
          if (!graph_ || graph_ntile_ != (int)tiles_.end() || graph_cfg_.waves != cfg_.waves ||
-             graph_cfg_.block != cfg_.block)
+             graph_cfg_.block != cfg_.block || graph_cfg_.occ != cfg_.occ)
             rebuild_graph();
          CDH_HIP(hipGraphLaunch(graph_, stream_));
          for (Slab &s : slabs_)

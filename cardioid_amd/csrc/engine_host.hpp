@@ -37,8 +37,10 @@ class RangeAlloc {
 };
 
 struct LaunchCfg {
-   int waves = 0;     // occupancy target of the kernel instance (waves per SIMD)
+   int waves = 0;     // register target of the kernel instance (waves per SIMD)
    int block = 256;   // threads per workgroup: 64, 128 or 256
+   int occ = 0;       // waves per SIMD allowed at run time (0: no cap), set
+                      // by reserving LDS per workgroup
 };
 
 class Engine {
@@ -70,6 +72,7 @@ class Engine {
 
    void run(int nsteps);
    double run_timed(int nsteps);          // ms between events around the steps
+   int lds_bytes(const LaunchCfg &c) const;
    void sync();
    void set_launch(const LaunchCfg &c);
    LaunchCfg launch() const { return cfg_; }
@@ -124,6 +127,7 @@ class Engine {
    int graph_ntile_ = -1;
    LaunchCfg graph_cfg_;
    int64_t bytes_ = 0;
+   int lds_per_cu_ = 0;
 
    RangeAlloc tiles_, boxes_, params_;
    std::vector<Slab> slabs_;

@@ -31,8 +31,18 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    const __amdgpu_buffer_rsrc_t R = box_rsrc(a.box_cur);
    const double V = box_load(R, b8, S.soff[0]);
 
-   // states and weights stream once per step (non-temporal); the tile's
-   // base addresses are wave-uniform
+   // the stencil first: its neighbour and weight loads are issued with the
+   // state loads at the start, so the reaction's arithmetic covers them all
+   const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
+   float acc = 0.0f;
+#pragma unroll
+   for (int k = 1; k < 19; k++)
+      acc = __builtin_fmaf(__builtin_nontemporal_load(wp + kBlk * (k - 1)),
+                           (float)(box_load(R, b8, S.soff[k]) - V), acc);
+   const double dVD = (double)acc * v.scale;
+
+   // states stream once per step (non-temporal); the tile's base addresses
+   // are wave-uniform
    float *p32 = a.st32 + (int64_t)blk * (N_STATE32 * kBlk) + lane;
    double *p64 = a.st64 + (int64_t)blk * (N_STATE64 * kBlk) + lane;
    float s32[N_STATE32];
@@ -47,13 +57,6 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    const double dVR = nongateM(*a.K, P, a.coef, a.dt, V, K_i, s32, s64, &iK);
    gatesM(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, s32, a.gate_mask);
 
-   const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
-   float acc = 0.0f;
-#pragma unroll
-   for (int k = 1; k < 19; k++)
-      acc = __builtin_fmaf(__builtin_nontemporal_load(wp + kBlk * (k - 1)),
-                           (float)(box_load(R, b8, S.soff[k]) - V), acc);
-   const double dVD = (double)acc * v.scale;
    engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);

@@ -183,8 +183,8 @@ def lib():
         L.cdh_batch_run_timed.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_sync.argtypes = [C.c_void_p]
         L.cdh_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
-        L.cdh_batch_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int]
-        L.cdh_batch_get_launch.argtypes = [C.c_void_p, ip, ip]
+        L.cdh_batch_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.cdh_batch_get_launch.argtypes = [C.c_void_p, ip, ip, ip]
         L.cdh_batch_autotune.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_int, ip]
         L.cdh_batch_get_vm.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_get_state.argtypes = [C.c_void_p, C.c_int, dp]
@@ -434,22 +434,25 @@ class Batch:
     def set_graph(self, steps: int):
         check(lib().cdh_batch_set_graph(self.h, int(steps)), "cdh_batch_set_graph")
 
-    def set_launch(self, waves: int, block: int):
-        check(lib().cdh_batch_set_launch(self.h, waves, block), "cdh_batch_set_launch")
+    def set_launch(self, waves: int, block: int, occupancy: int = 0):
+        check(lib().cdh_batch_set_launch(self.h, waves, block, occupancy), "cdh_batch_set_launch")
 
     def launch(self):
-        w, b = C.c_int(), C.c_int()
-        check(lib().cdh_batch_get_launch(self.h, C.byref(w), C.byref(b)), "cdh_batch_get_launch")
-        return w.value, b.value
+        """(register target, workgroup size, occupancy cap)"""
+        w, b, o = C.c_int(), C.c_int(), C.c_int()
+        check(lib().cdh_batch_get_launch(self.h, C.byref(w), C.byref(b), C.byref(o)),
+              "cdh_batch_get_launch")
+        return w.value, b.value, o.value
 
     def autotune(self, steps: int = 10):
+        """(best ms per step, [(waves, block, occupancy, ms per step), ...])"""
         best = C.c_double()
-        table = np.zeros(3 * 64)
+        table = np.zeros(4 * 64)
         nrows = C.c_int()
         check(lib().cdh_batch_autotune(self.h, int(steps), C.byref(best), ptr(table), 64,
                                        C.byref(nrows)), "cdh_batch_autotune")
-        rows = [(int(table[3 * k]), int(table[3 * k + 1]), float(table[3 * k + 2]))
-                for k in range(min(nrows.value, 64))]
+        rows = [(int(table[4 * k]), int(table[4 * k + 1]), int(table[4 * k + 2]),
+                 float(table[4 * k + 3])) for k in range(min(nrows.value, 64))]
         return best.value, rows
 
     def vm(self, sid: int) -> np.ndarray:

@@ -1,0 +1,124 @@
+"""Summarises rocprofv3 results of one kernel into a short markdown file and
+a traffic JSON (used by bench.py for roofline.traffic).
+
+    python -m cardioid_amd.profile_summary --kernel NAME --cells N --bytes B \
+        --trace DIR --fetch DIR --write DIR --out profiles/STEM \
+        --title "..." --cmd-trace "..." --cmd-fetch "..." --cmd-write "..."
+
+DIR are rocprofv3 output directories (SQLite results).  FETCH_SIZE and
+WRITE_SIZE (KiB per dispatch) are scaled by what the same counters read for
+the 512 MiB device-to-device copy that the profiled program runs first
+(`__amd_rocclr_copyBuffer`: 512 MiB read and 512 MiB written).
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import json
+import os
+import sqlite3
+
+COPY_BYTES = 512 * 2 ** 20     # per copy; profile_step makes 3 (the runtime may split each)
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def _db(d):
+    f = sorted(glob.glob(os.path.join(d, "**", "*.db"), recursive=True))
+    if not f:
+        raise FileNotFoundError(f"no rocprofv3 database under {d}")
+    return sqlite3.connect(f[0])
+
+
+def trace_rows(d):
+    c = _db(d)
+    return [tuple(r) for r in c.execute(
+        "select name, total_calls, total_duration, average, percentage from top_kernels")]
+
+
+def counter_means(d, counter):
+    """kernel name -> (mean per dispatch, dispatches, sum over dispatches)"""
+    c = _db(d)
+    rows = c.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                     "where counter_name = ? group by kernel_name, dispatch_id", (counter,))
+    acc = {}
+    for name, _, v in rows:
+        acc.setdefault(name, []).append(v)
+    return {k: (sum(v) / len(v), len(v), sum(v)) for k, v in acc.items()}
+
+
+def _pick(d, kernel):
+    keys = [k for k in d if kernel in k]
+    if not keys:
+        raise KeyError(f"{kernel} not in {list(d)}")
+    return keys[0]
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--kernel", required=True)
+    ap.add_argument("--cells", type=int, required=True)
+    ap.add_argument("--bytes", type=float, required=True, help="algorithmic bytes per cell-step")
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--title", required=True)
+    ap.add_argument("--workload", default="")
+    ap.add_argument("--cmd-trace", default="")
+    ap.add_argument("--cmd-fetch", default="")
+    ap.add_argument("--cmd-write", default="")
+    ap.add_argument("--note", default="")
+    ap.add_argument("--copies", type=int, default=3)
+    a = ap.parse_args(argv)
+
+    rows = trace_rows(a.trace)
+    k = [r for r in rows if a.kernel in r[0]][0]
+    avg_us = k[3]
+    lines = [f"# {a.title}", ""]
+    if a.note:
+        lines += [a.note, ""]
+    lines += ["## Kernel trace", "", "Command (from the repo root, on the GPU box):", "",
+              f"    {a.cmd_trace}", "", "| kernel | calls | total (us) | average (us) | % |",
+              "|---|---|---|---|---|"]
+    for r in rows[:6]:
+        lines.append(f"| `{r[0][:90]}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.2f} |")
+    achieved = a.bytes * a.cells / (avg_us * 1e-6) / 1e9
+    lines += ["", f"Average duration of `{a.kernel}`: {avg_us:.1f} us per launch (one step of "
+              f"{a.cells} tissue cells) = {a.cells / avg_us / 1e3:.3f} Gcell-steps/s; at "
+              f"{a.bytes:g} algorithmic bytes per cell-step that is {achieved:.0f} GB/s "
+              f"({achieved / 8000:.3f} of 8 TB/s).", ""]
+    out = {"kernel": k[0], "workload": a.workload, "tissue_cells": a.cells,
+           "average_us": avg_us, "algorithmic_bytes_per_cell_step": a.bytes}
+    if a.fetch and a.write:
+        f = counter_means(a.fetch, "FETCH_SIZE")
+        w = counter_means(a.write, "WRITE_SIZE")
+        cf, cw = f[_pick(f, COPY_KERNEL)], w[_pick(w, COPY_KERNEL)]
+        fs = a.copies * COPY_BYTES / (cf[2] * 1024)
+        ws = a.copies * COPY_BYTES / (cw[2] * 1024)
+        fk, nf, _ = f[_pick(f, a.kernel)]
+        wk, nw, _ = w[_pick(w, a.kernel)]
+        fb, wb = fk * 1024 * fs, wk * 1024 * ws
+        lines += ["## HBM traffic per launch (FETCH_SIZE, WRITE_SIZE)", "",
+                  "Commands (one counter per run):", "", f"    {a.cmd_fetch}", f"    {a.cmd_write}",
+                  "", f"Calibration: the {a.copies} device-to-device copies of 512 MiB at the start "
+                  f"of the same runs ({cf[1]} dispatches of `{COPY_KERNEL}`) read FETCH_SIZE = "
+                  f"{cf[2] / a.copies:.1f} KiB and WRITE_SIZE = {cw[2] / a.copies:.1f} KiB per copy, "
+                  f"so bytes read = FETCH_SIZE x 1024 x {fs:.4f} and bytes written = WRITE_SIZE x "
+                  f"1024 x {ws:.4f}.", "",
+                  "| | raw (KiB/launch) | launches | scaled bytes/launch | bytes per tissue cell-step |",
+                  "|---|---|---|---|---|",
+                  f"| FETCH_SIZE | {fk:.1f} | {nf} | {fb:.4e} | {fb / a.cells:.1f} |",
+                  f"| WRITE_SIZE | {wk:.1f} | {nw} | {wb:.4e} | {wb / a.cells:.1f} |",
+                  f"| total | | | {fb + wb:.4e} | {(fb + wb) / a.cells:.1f} "
+                  f"(algorithmic {a.bytes:g}) |", ""]
+        out.update(fetch_bytes_per_launch=fb, write_bytes_per_launch=wb,
+                   bytes_per_launch=fb + wb, fetch_scale=fs, write_scale=ws)
+    with open(a.out + ".md", "w") as fh:
+        fh.write("\n".join(lines))
+    with open(a.out + "_traffic.json", "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()

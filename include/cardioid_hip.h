@@ -289,13 +289,15 @@ int cdh_batch_run_timed(cdh_batch *b, int nsteps, double *ms);
 int cdh_batch_sync(cdh_batch *b);
 /* steps per captured HIP graph (even; 0 launches each step directly) */
 int cdh_batch_set_graph(cdh_batch *b, int steps);
-/* kernel occupancy target (waves/SIMD: FP64 2..4, mixed 3..5) and workgroup
- * size (64, 128, 256) */
-int cdh_batch_set_launch(cdh_batch *b, int waves, int block);
-int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block);
+/* launch configuration: the kernel instance's register target (waves/SIMD:
+ * FP64 2..4, mixed 3..5), the workgroup size (64, 128, 256) and a cap on
+ * the waves per SIMD resident at run time (0: none; applied by reserving
+ * LDS per workgroup) */
+int cdh_batch_set_launch(cdh_batch *b, int waves, int block, int occupancy);
+int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block, int *occupancy);
 /* times every launch configuration for `steps` steps from the current state
  * (which is restored afterwards) and keeps the fastest; the table gets
- * (waves, block, ms per step) rows, up to max_rows */
+ * (waves, block, occupancy, ms per step) rows, up to max_rows */
 int cdh_batch_autotune(cdh_batch *b, int steps, double *best_ms, double *table, int max_rows,
                        int *nrows);
 /* per-slab results, in the slab's input cell order */

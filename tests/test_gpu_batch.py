@@ -170,8 +170,8 @@ def test_autotune_restores_state_and_picks_a_candidate():
         if tune:
             best, rows = b.autotune(5)
             say(f"  autotune: best {best:.4f} ms/step, launch {b.launch()}, {len(rows)} candidates")
-            assert len(rows) == 9 and best > 0
-            assert b.launch() in [(r[0], r[1]) for r in rows]
+            assert len(rows) == 18 and best > 0
+            assert b.launch() in [(r[0], r[1], r[2]) for r in rows]
         b.run(250)
         res.append([np.concatenate([b.vm(i), b.state(i).ravel(), b.t90(i)]) for i in ids])
         assert [b.slab_info(i)["loop"] for i in ids] == [300] * 3
