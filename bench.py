@@ -161,6 +161,7 @@ def main():
     ap.add_argument("--grid", type=str, default="256,256,512")
     ap.add_argument("--unfused-steps", type=int, default=500)
     ap.add_argument("--mixed-steps", type=int, default=5000)
+    ap.add_argument("--exact-steps", type=int, default=2000)
     ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--graph", type=int, default=10)
     ap.add_argument("--batched-steps", type=int, default=2000)
@@ -186,7 +187,10 @@ def main():
     slab = hip.Slab(anat, W.CELLTYPES_EMI, W.DIFFUSION_SCALE, stims)
 
     res = {}
-    for mode, steps in (("fp64_fused", args.steps), ("mixed_fused", args.mixed_steps)):
+    for mode, steps in (("fp64_fma", args.steps), ("mixed_fused", args.mixed_steps),
+                        ("fp64_fused", args.exact_steps)):
+        if steps <= 0:
+            continue
         t = time.perf_counter()
         r = fused_run(hip, slab, anat.ncell, mode, steps, args.warmup, args.tune_steps, args.graph)
         res[mode] = r
@@ -203,7 +207,7 @@ def main():
                                    rate=anat.ncell * args.unfused_steps / (ms * 1e-3) / 1e9)
         log(f"fp64_unfused: {res['fp64_unfused']['rate']:.3f} Gcell-steps/s")
 
-    f64, mx = res["fp64_fused"], res["mixed_fused"]
+    f64, mx = res["fp64_fma"], res["mixed_fused"]
     ms_per_step = f64["ms"] / f64["steps"]
     achieved = FUSED_F64_BYTES * anat.ncell / (ms_per_step * 1e-3) / 1e9
     out = {
@@ -221,22 +225,23 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"ellipsoidal shell {nx}x{ny}x{nz}, ~60% tissue, endo/mid/epi, "
                                "rotating fibres, 8 endocardial stimuli, dt 0.02 ms, plain FP64, "
-                               "fused step",
+                               "fused step (fit polynomials with FMA; Vm parity with the "
+                               "reference path within 1e-9)",
                    "grid_points": nx * ny * nz, "tissue_cells": anat.ncell, "dt_ms": DT,
-                   "mode": "fp64_fused",
+                   "mode": "fp64_fma",
                    "launch": {"waves": f64["launch"][0], "block": f64["launch"][1],
                               "occupancy": f64["launch"][2]},
                    "graph_steps": args.graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic("engine_f64_shell256x256x512"),
+                     "traffic": load_traffic("engine_f64_fma_shell256x256x512"),
                      "kernel": "k_engine_f64", "algorithmic_bytes_per_cell_step": FUSED_F64_BYTES},
     }
     mms = mx["ms"] / mx["steps"]
     both = (f64["act"] >= 0) & (mx["act"] >= 0)
     out["mixed_fused"] = {
         "value": mx["rate"], "unit": "Gcell-steps/s", "steps": mx["steps"], "ms_per_step": mms,
-        "speedup_vs_fp64_fused": mx["rate"] / f64["rate"],
+        "speedup_vs_fp64": mx["rate"] / f64["rate"],
         "launch": {"waves": mx["launch"][0], "block": mx["launch"][1],
                    "occupancy": mx["launch"][2]},
         "algorithmic_bytes_per_cell_step": MIXED_BYTES,
@@ -250,6 +255,15 @@ def main():
             "activated_same_cells": bool(np.array_equal(f64["act"] >= 0, mx["act"] >= 0))}
     out["range_check"] = {"fp64_out_of_range": (f64["range"] or {}).get("count", 0),
                           "mixed_out_of_range": (mx["range"] or {}).get("count", 0)}
+    if "fp64_fused" in res:
+        ex = res["fp64_fused"]
+        out["fp64_reference_rounding"] = {
+            "value": ex["rate"], "unit": "Gcell-steps/s", "steps": ex["steps"],
+            "ms_per_step": ex["ms"] / ex["steps"],
+            "launch": {"waves": ex["launch"][0], "block": ex["launch"][1],
+                       "occupancy": ex["launch"][2]},
+            "note": "no FMA anywhere: bit for bit the unfused step's rounding",
+            "mixed_speedup": mx["rate"] / ex["rate"]}
     if "fp64_unfused" in res:
         u = res["fp64_unfused"]
         out["fp64_unfused"] = {"value": u["rate"], "unit": "Gcell-steps/s", "steps": u["steps"],

@@ -89,7 +89,7 @@ struct EngineArgs {
    int ntile;                   // 64-cell tiles in use
    int groups_per_xcd;          // workgroup tiles per XCD
    double dt;
-   uint32_t gate_mask;          // all 12 bits set; see gates64
+   uint32_t gate_mask;          // bits 0..12 set; see gates64 and the kernels
    int sensors;                 // bit 0: activation times, bit 1: APD90
    double *tact0, *tact, *trec;
    uint8_t *apd_phase;
@@ -214,7 +214,7 @@ __device__ __forceinline__ double box_load(__amdgpu_buffer_rsrc_t r, uint32_t vo
 }
 
 // kernels (engine_f64.hip, engine_mixed.hip); WAVES = target waves per SIMD
-template <int WAVES>
+template <int WAVES, bool FMA>
 __global__ void k_engine_f64(EngineArgs a);
 template <int WAVES>
 __global__ void k_engine_mixed(EngineArgs a);

@@ -1,10 +1,12 @@
 // engine_f64.hip -- the fused FP64 step over the engine's slabs (gfx950).
 //
-// Compiled with -ffp-contract=off: the reaction, the stencil and the Vm
-// update round as the reference CPU path does, operation by operation
-// (TT06Dev_Reaction::calc, elec/TT06Dev_Reaction.cc:393-407;
+// Compiled with -ffp-contract=off: with FMA = false the reaction, the
+// stencil and the Vm update round as the reference CPU path does, operation
+// by operation (TT06Dev_Reaction::calc, elec/TT06Dev_Reaction.cc:393-407;
 // FGRDiffusionOMP::calc, elec/FGRDiffusionOMP.cc:109-131; the integrator,
-// elec/simulationLoop.cc:221-227).
+// elec/simulationLoop.cc:221-227).  FMA = true evaluates the 33 fit
+// polynomials with fused multiply-adds (half the instructions of the
+// multiply-add pairs) and leaves everything else as it is.
 #pragma clang fp contract(off)
 
 #include "engine.hpp"
@@ -13,6 +15,7 @@ namespace cdh {
 
 namespace {
 
+template <bool FMA>
 __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, int sl,
                                          const SlabView &v, int blk, int lane)
 {
@@ -44,8 +47,12 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 #pragma unroll
    for (int s = 0; s < N_STATE; s++) st[s] = __builtin_nontemporal_load(sp + kBlk * s);
    const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
-   const double dVR = nongate64(*a.K, P, a.coef, a.dt, V, st);
-   gates64(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, st, a.gate_mask);
+   // the non-gate update in a basic block of its own (gate_mask bit 12,
+   // always set): otherwise the compiler hoists its scalar coefficient loads
+   // to the kernel entry and spills them
+   double dVR = 0.0;
+   if (a.gate_mask & (1u << 12)) dVR = nongate64<FMA>(*a.K, P, a.coef, a.dt, V, st);
+   gates64<FMA>(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, st, a.gate_mask);
 #pragma unroll
    for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
 
@@ -63,7 +70,7 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 // Workgroups are dealt round-robin to the 8 XCDs; workgroup w serves a tile
 // group in the contiguous range of XCD (w mod 8), so that each XCD's L2
 // holds the box rows its cells read.  The grid is 8 * groups_per_xcd.
-template <int WAVES>
+template <int WAVES, bool FMA>
 __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
 {
    const int xcd = blockIdx.x & 7;
@@ -78,12 +85,13 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
    if (!S.active) return;
    SlabView v;
    slab_view(S, a.parity, a.dt, v);
-   cell_f64(a, S, sl, v, blk, lane);
+   cell_f64<FMA>(a, S, sl, v, blk, lane);
    if (blk == S.first_blk && lane == 0) slab_advance(S, a.parity, v);
 }
 
-template __global__ void k_engine_f64<2>(EngineArgs);
-template __global__ void k_engine_f64<3>(EngineArgs);
-template __global__ void k_engine_f64<4>(EngineArgs);
+template __global__ void k_engine_f64<2, false>(EngineArgs);
+template __global__ void k_engine_f64<3, false>(EngineArgs);
+template __global__ void k_engine_f64<2, true>(EngineArgs);
+template __global__ void k_engine_f64<3, true>(EngineArgs);
 
 }  // namespace cdh

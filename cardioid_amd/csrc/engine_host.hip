@@ -78,18 +78,14 @@ using KernelFn = void (*)(EngineArgs);
 KernelFn kernel_for(int mode, int waves)
 {
    if (mode == CDH_MODE_MIXED_FUSED) {
-      switch (waves) {
-      case 4: return k_engine_mixed<4>;
-      case 5: return k_engine_mixed<5>;
-      default: return k_engine_mixed<3>;
-      }
+      return waves == 4 ? k_engine_mixed<4> : k_engine_mixed<3>;
    }
-   switch (waves) {
-   case 2: return k_engine_f64<2>;
-   case 4: return k_engine_f64<4>;
-   default: return k_engine_f64<3>;
-   }
+   if (mode == CDH_MODE_FP64_FMA) return waves == 2 ? k_engine_f64<2, true> : k_engine_f64<3, true>;
+   return waves == 2 ? k_engine_f64<2, false> : k_engine_f64<3, false>;
 }
+
+int min_waves(int mode) { return mode == CDH_MODE_MIXED_FUSED ? 3 : 2; }
+int max_waves(int mode) { return mode == CDH_MODE_MIXED_FUSED ? 4 : 3; }
 
 // a slab's [nq][n] array (slab order) from its blocked device range
 template <class T>
@@ -121,7 +117,7 @@ void download(const T *dev, T *host, size_t count)
 std::vector<LaunchCfg> Engine::candidates(int mode)
 {
    std::vector<LaunchCfg> c;
-   const int w0 = mode == CDH_MODE_MIXED_FUSED ? 3 : 2;
+   const int w0 = min_waves(mode);
    for (int w = w0; w <= w0 + 1; w++)
       for (int b : {64, 128, 256})
          for (int o : {0, 2, 3}) c.push_back(LaunchCfg{w, b, o});
@@ -168,8 +164,9 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
                int64_t cap_cells, int64_t cap_box, int cap_slabs, int cap_params, int sensors)
     : mode_(mode), dt_(dt), sensors_(sensors)
 {
-   CDH_REQUIRE(mode == CDH_MODE_FP64_FUSED || mode == CDH_MODE_MIXED_FUSED,
-               "engine: mode must be FP64 fused or mixed fused");
+   CDH_REQUIRE(mode == CDH_MODE_FP64_FUSED || mode == CDH_MODE_FP64_FMA ||
+                   mode == CDH_MODE_MIXED_FUSED,
+               "engine: mode must be FP64 fused, FP64 FMA or mixed fused");
    CDH_REQUIRE(dt > 0, "engine: dt must be positive");
    CDH_REQUIRE(fit_l && fit_m && fit_coef, "engine: null fit arrays");
    CDH_REQUIRE(cap_cells >= 1 && cap_cells < (int64_t(1) << 31) - 64,
@@ -210,7 +207,7 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    slabdev_.zero();
    track(ranges_, cap_slabs);
    ranges_.zero();
-   if (mode == CDH_MODE_FP64_FUSED) {
+   if (mode != CDH_MODE_MIXED_FUSED) {
       track(state_, (size_t)N_STATE * S);
       track(A0_, S);
    } else {
@@ -315,7 +312,7 @@ int Engine::add(const SlabHost &h)
    std::vector<uint32_t> bi(S);
    for (int64_t k = 0; k < S; k++) bi[k] = (uint32_t)(b0 + h.bi[k]);
    put(bi_.p + c0, bi);
-   if (mode_ == CDH_MODE_FP64_FUSED) {
+   if (mode_ != CDH_MODE_MIXED_FUSED) {
       put(state_.p + c0 * N_STATE, block(h.state, N_STATE, S, S));
       put(A0_.p + c0, h.A0);
    } else {
@@ -406,7 +403,7 @@ EngineArgs Engine::args(int parity) const
    const int ngroup = (a.ntile + wpg - 1) / wpg;
    a.groups_per_xcd = (ngroup + 7) / 8;
    a.dt = dt_;
-   a.gate_mask = 0xFFFu;
+   a.gate_mask = 0x1FFFu;
    a.sensors = sensors_;
    a.tact0 = tact0_.p;
    a.tact = tact_.p;
@@ -474,8 +471,8 @@ void Engine::set_graph_steps(int k)
 
 void Engine::set_launch(const LaunchCfg &c)
 {
-   const int w0 = mode_ == CDH_MODE_MIXED_FUSED ? 3 : 2;
-   CDH_REQUIRE(c.waves >= w0 && c.waves <= w0 + 2, "engine: no kernel for that occupancy");
+   CDH_REQUIRE(c.waves >= min_waves(mode_) && c.waves <= max_waves(mode_),
+               "engine: no kernel instance for that register target");
    CDH_REQUIRE(c.block == 64 || c.block == 128 || c.block == 256,
                "engine: workgroup size must be 64, 128 or 256");
    CDH_REQUIRE(c.occ >= 0 && c.occ <= 8, "engine: occupancy cap must be 0..8");
@@ -612,7 +609,7 @@ std::vector<double> Engine::state19(int id)
    sync();
    const Slab &s = slabs_[id];
    const int64_t S = s.ntile * kBlk, c0 = s.tile0 * kBlk, n = s.n;
-   if (mode_ == CDH_MODE_FP64_FUSED) {
+   if (mode_ != CDH_MODE_MIXED_FUSED) {
       std::vector<double> b((size_t)N_STATE * S);
       download(state_.p + c0 * N_STATE, b.data(), b.size());
       return unblock(b, N_STATE, n);
@@ -645,7 +642,7 @@ void Engine::set_state19(int id, const std::vector<double> &st, const std::vecto
    const Slab &s = slabs_[id];
    const int64_t S = s.ntile * kBlk, c0 = s.tile0 * kBlk, n = s.n;
    // padding cells keep their states: start from the device copy
-   if (mode_ == CDH_MODE_FP64_FUSED) {
+   if (mode_ != CDH_MODE_MIXED_FUSED) {
       std::vector<double> b((size_t)N_STATE * S);
       download(state_.p + c0 * N_STATE, b.data(), b.size());
       for (int q = 0; q < N_STATE; q++)

@@ -54,7 +54,9 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
    const double K_i = P.K0 + (double)s32[M_DK_I];
    double iK;
-   const double dVR = nongateM(*a.K, P, a.coef, a.dt, V, K_i, s32, s64, &iK);
+   double dVR = 0.0;
+   iK = 0.0;
+   if (a.gate_mask & (1u << 12)) dVR = nongateM(*a.K, P, a.coef, a.dt, V, K_i, s32, s64, &iK);
    gatesM(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, s32, a.gate_mask);
 
    engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
@@ -96,6 +98,5 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_mixed(EngineArgs a)
 
 template __global__ void k_engine_mixed<3>(EngineArgs);
 template __global__ void k_engine_mixed<4>(EngineArgs);
-template __global__ void k_engine_mixed<5>(EngineArgs);
 
 }  // namespace cdh

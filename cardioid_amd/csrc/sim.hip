@@ -324,18 +324,18 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
       CDH_REQUIRE(nstim >= 0 && nstim <= kMaxStim && (nstim == 0 || stim),
                   "cdh_sim_create: at most 8 stimuli");
       CDH_REQUIRE(p->mode == CDH_MODE_FP64_UNFUSED || p->mode == CDH_MODE_FP64_FUSED ||
-                      p->mode == CDH_MODE_MIXED_FUSED,
+                      p->mode == CDH_MODE_MIXED_FUSED || p->mode == CDH_MODE_FP64_FMA,
                   "cdh_sim_create: unknown mode");
       CDH_REQUIRE(p->dt > 0, "cdh_sim_create: dt must be positive");
       CDH_REQUIRE(p->fit_l && p->fit_m && p->fit_coef, "cdh_sim_create: null fit arrays");
       CDH_REQUIRE(!(p->sensors & 2) || (p->sensors & 1),
                   "cdh_sim_create: APD90 needs the activation sensors (sensors bit 0)");
       if (p->mode == CDH_MODE_MIXED_FUSED)
-         CDH_REQUIRE(p->variant == 0 || (p->variant >= 3 && p->variant <= 5),
-                     "cdh_sim_create: mixed variant must be 0 (default) or 3..5");
+         CDH_REQUIRE(p->variant == 0 || (p->variant >= 3 && p->variant <= 4),
+                     "cdh_sim_create: mixed variant must be 0 (default) or 3..4");
       else
-         CDH_REQUIRE(p->variant == 0 || (p->variant >= 2 && p->variant <= 4),
-                     "cdh_sim_create: variant must be 0 (default) or 2..4");
+         CDH_REQUIRE(p->variant == 0 || (p->variant >= 2 && p->variant <= 3),
+                     "cdh_sim_create: variant must be 0 (default) or 2..3");
       auto s = std::make_unique<cdh_sim>();
       s->p = *p;
       s->mode = p->mode;

@@ -29,27 +29,55 @@ struct ParamSet {
 
 constexpr int kMaxParamSets = 256;
 
-template <int LO, int HI, class P>
+// x*s + a with the coefficient a taken from SGPRs by the VOP3 encoding.
+// Written out because the compiler otherwise picks v_fmac_f64 (addend tied
+// to the destination) and copies every coefficient into VGPRs first, two
+// v_mov per Horner step.
+__device__ __forceinline__ double fma_vvs(double x, double s, double a)
+{
+   double r;
+   asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(s), "s"(a));
+   return r;
+}
+
+// Horner of a[LO..HI] at x.  FMA = false: separate multiply and add, the
+// reference's rounding (pade.cc:17-36 built without contraction).  FMA =
+// true: fused multiply-adds; the first step stays a multiply and an add,
+// which take their coefficient straight from SGPRs (a first FMA would need
+// one of its two coefficients copied into VGPRs: one scalar operand per
+// instruction).
+template <int LO, int HI, bool FMA = false, class P>
 __device__ __forceinline__ double horner64(P a, double x)
 {
-   double s = a[HI];
+   if constexpr (FMA && HI > LO) {
+      double s;
+      {
+#pragma clang fp contract(off)
+         s = x * a[HI] + a[HI - 1];
+      }
 #pragma unroll
-   for (int j = HI - 1; j >= LO; --j) s = a[j] + x * s;
-   return s;
+      for (int j = HI - 2; j >= LO; --j) s = fma_vvs(x, s, a[j]);
+      return s;
+   } else {
+      double s = a[HI];
+#pragma unroll
+      for (int j = HI - 1; j >= LO; --j) s = a[j] + x * s;
+      return s;
+   }
 }
 
 // rational fit i of the fixed layout (pade.cc:17-36: num/den, polynomial if l < 2)
-template <int I>
+template <int I, bool FMA = false>
 __device__ __forceinline__ double fit64(const double *__restrict__ coef, double x)
 {
    constexpr int off = fit_offset(I);
    constexpr int l = kFitOrder[I].l, m = kFitOrder[I].m;
    const double *__restrict__ c = coef + off;
-   const double num = horner64<0, m - 1>(c, x);
+   const double num = horner64<0, m - 1, FMA>(c, x);
    if constexpr (l < 2) {
       return num;
    } else {
-      const double den = horner64<m, m + l - 1>(c, x);
+      const double den = horner64<m, m + l - 1, FMA>(c, x);
       return num / den;
    }
 }
@@ -57,14 +85,14 @@ __device__ __forceinline__ double fit64(const double *__restrict__ coef, double 
 enum GateForm { G_RL, G_DIF, G_SUM };
 
 // one gate: mhu and dt/tau from fits 7+2*eq and 8+2*eq, then the v2 update
-template <int EQ, GateForm FORM>
+template <int EQ, GateForm FORM, bool FMA = false>
 __device__ __forceinline__ double gate64(const double *__restrict__ coef, const double *__restrict__ expc,
                                          double V, double g, bool sub_form)
 {
-   const double mhu = fit64<7 + 2 * EQ>(coef, V);
-   const double x = fit64<8 + 2 * EQ>(coef, V);
+   const double mhu = fit64<7 + 2 * EQ, FMA>(coef, V);
+   const double x = fit64<8 + 2 * EQ, FMA>(coef, V);
    if constexpr (FORM == G_RL) {
-      const double e = horner64<0, 16>(expc, x);
+      const double e = horner64<0, 16, FMA>(expc, x);
       const double r = 1.0 - 1.0 / e;
       return r * (mhu - g) + g;
    } else if constexpr (FORM == G_SUM) {
@@ -78,13 +106,16 @@ __device__ __forceinline__ double gate64(const double *__restrict__ coef, const 
 #define CDH_SIGM(x) ((x) / (1.0 + (x)))
 
 // Non-gate update of one cell; returns dVdt = -Iion and updates the seven
-// concentration-like states in st[].  Gates are read, not written.
+// concentration-like states in st[].  Gates are read, not written.  FMA
+// applies to the fit polynomials only.
+template <bool FMA = false>
 __device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &P,
                                            const double *__restrict__ coef, double dt, double Vm,
                                            double *st)
 {
-   const double fv0 = fit64<0>(coef, Vm), fv1 = fit64<1>(coef, Vm), fv2 = fit64<2>(coef, Vm);
-   const double fv3 = fit64<3>(coef, Vm), fv4 = fit64<4>(coef, Vm), fv5 = fit64<5>(coef, Vm);
+   const double fv0 = fit64<0, FMA>(coef, Vm), fv1 = fit64<1, FMA>(coef, Vm);
+   const double fv2 = fit64<2, FMA>(coef, Vm), fv3 = fit64<3, FMA>(coef, Vm);
+   const double fv4 = fit64<4, FMA>(coef, Vm), fv5 = fit64<5, FMA>(coef, Vm);
    const double Na_i = st[S_NA_I], Ca_i = st[S_CA_I], Ca_ss = st[S_CA_SS], Ca_SR = st[S_CA_SR];
    const double fCass = st[S_FCASS], R_prime = st[S_R_PRIME];
    double dVK_i = st[S_DVK_I];
@@ -116,7 +147,7 @@ __device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &
    const double dV0 = Vm - K.c3 * log(K_i) - K.c5;
    const double dV1 = Vm - K.c3 * log(Na_i) - K.c4;
    const double dV2 = dV0 - K.c3 * log(1 + K.c2 * Na_i / K_i) + K.c5 - K.c6;
-   const double fv6 = fit64<6>(coef, dV0);
+   const double fv6 = fit64<6, FMA>(coef, dV0);
    const double m = st[S_M], h = st[S_H], j = st[S_J];
    const double I_NaK = c_NaK * CDH_SIGM(x0) * fv0;
    const double I_pK = c_pK * fv5 * dV0;
@@ -161,26 +192,27 @@ __device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &
 // each gate in a basic block of its own, which keeps the compiler from
 // issuing the scalar loads of all 26 gate fits ahead of time (they would
 // not fit in SGPRs and get spilled to VGPR lanes).
+template <bool FMA = false>
 __device__ __forceinline__ void gates64(const double *__restrict__ coef, const double *__restrict__ expc,
                                         int s_switch, uint8_t flags, double Vm, double *st,
                                         uint32_t gmask)
 {
    if (flags & kFlagGatesFixed) return;
    const bool sub = (flags & kFlagFGateSub) != 0;
-   if (gmask & (1u << 0)) st[S_M] = gate64<0, G_RL>(coef, expc, Vm, st[S_M], false);
-   if (gmask & (1u << 1)) st[S_H] = gate64<1, G_DIF>(coef, expc, Vm, st[S_H], false);
-   if (gmask & (1u << 2)) st[S_J] = gate64<2, G_DIF>(coef, expc, Vm, st[S_J], false);
-   if (gmask & (1u << 3)) st[S_XR1] = gate64<3, G_DIF>(coef, expc, Vm, st[S_XR1], false);
-   if (gmask & (1u << 4)) st[S_XR2] = gate64<4, G_SUM>(coef, expc, Vm, st[S_XR2], false);
-   if (gmask & (1u << 5)) st[S_XS] = gate64<5, G_DIF>(coef, expc, Vm, st[S_XS], false);
-   if (gmask & (1u << 6)) st[S_R] = gate64<6, G_DIF>(coef, expc, Vm, st[S_R], false);
-   if (gmask & (1u << 7)) st[S_D] = gate64<7, G_DIF>(coef, expc, Vm, st[S_D], false);
-   if (gmask & (1u << 8)) st[S_F] = gate64<8, G_DIF>(coef, expc, Vm, st[S_F], sub);
-   if (gmask & (1u << 9)) st[S_F2] = gate64<9, G_DIF>(coef, expc, Vm, st[S_F2], false);
-   if (gmask & (1u << 10)) st[S_JL] = gate64<10, G_DIF>(coef, expc, Vm, st[S_JL], false);
+   if (gmask & (1u << 0)) st[S_M] = gate64<0, G_RL, FMA>(coef, expc, Vm, st[S_M], false);
+   if (gmask & (1u << 1)) st[S_H] = gate64<1, G_DIF, FMA>(coef, expc, Vm, st[S_H], false);
+   if (gmask & (1u << 2)) st[S_J] = gate64<2, G_DIF, FMA>(coef, expc, Vm, st[S_J], false);
+   if (gmask & (1u << 3)) st[S_XR1] = gate64<3, G_DIF, FMA>(coef, expc, Vm, st[S_XR1], false);
+   if (gmask & (1u << 4)) st[S_XR2] = gate64<4, G_SUM, FMA>(coef, expc, Vm, st[S_XR2], false);
+   if (gmask & (1u << 5)) st[S_XS] = gate64<5, G_DIF, FMA>(coef, expc, Vm, st[S_XS], false);
+   if (gmask & (1u << 6)) st[S_R] = gate64<6, G_DIF, FMA>(coef, expc, Vm, st[S_R], false);
+   if (gmask & (1u << 7)) st[S_D] = gate64<7, G_DIF, FMA>(coef, expc, Vm, st[S_D], false);
+   if (gmask & (1u << 8)) st[S_F] = gate64<8, G_DIF, FMA>(coef, expc, Vm, st[S_F], sub);
+   if (gmask & (1u << 9)) st[S_F2] = gate64<9, G_DIF, FMA>(coef, expc, Vm, st[S_F2], false);
+   if (gmask & (1u << 10)) st[S_JL] = gate64<10, G_DIF, FMA>(coef, expc, Vm, st[S_JL], false);
    if (gmask & (1u << 11)) {
-      if (s_switch) st[S_S] = gate64<12, G_DIF>(coef, expc, Vm, st[S_S], false);
-      else          st[S_S] = gate64<11, G_DIF>(coef, expc, Vm, st[S_S], false);
+      if (s_switch) st[S_S] = gate64<12, G_DIF, FMA>(coef, expc, Vm, st[S_S], false);
+      else          st[S_S] = gate64<11, G_DIF, FMA>(coef, expc, Vm, st[S_S], false);
    }
 }
 

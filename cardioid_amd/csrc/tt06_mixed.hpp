@@ -30,35 +30,11 @@ enum Slot32 : int {
 };
 enum Slot64 : int { M_CA_I = 0, M_NA_I, N_STATE64 };
 
-// x*s + a with the coefficient a taken from SGPRs by the VOP3 encoding.
-// Written out because the compiler otherwise picks v_fmac_f64 (addend tied
-// to the destination) and copies every coefficient into VGPRs first, two
-// v_mov per Horner step.
-__device__ __forceinline__ double fma_vvs(double x, double s, double a)
-{
-   double r;
-   asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(s), "s"(a));
-   return r;
-}
-
-// Horner with FMA.  The first step is a multiply and an add, which take
-// their coefficient straight from SGPRs; a first FMA would need one of its
-// two coefficients copied into VGPRs (one scalar operand per instruction).
+// Horner with fused multiply-adds (tt06_f64.hpp)
 template <int LO, int HI>
 __device__ __forceinline__ double hornerFma(const double *__restrict__ a, double x)
 {
-   if constexpr (HI == LO) {
-      return a[HI];
-   } else {
-      double s;
-      {
-#pragma clang fp contract(off)
-         s = x * a[HI] + a[HI - 1];
-      }
-#pragma unroll
-      for (int j = HI - 2; j >= LO; --j) s = fma_vvs(x, s, a[j]);
-      return s;
-   }
+   return horner64<LO, HI, true>(a, x);
 }
 
 __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }

@@ -20,8 +20,9 @@ NSTATE = 19
 MODE_FP64_UNFUSED = 0
 MODE_FP64_FUSED = 1
 MODE_MIXED_FUSED = 2
+MODE_FP64_FMA = 3
 MODES = {"fp64_unfused": MODE_FP64_UNFUSED, "fp64_fused": MODE_FP64_FUSED,
-         "mixed_fused": MODE_MIXED_FUSED}
+         "mixed_fused": MODE_MIXED_FUSED, "fp64_fma": MODE_FP64_FMA}
 
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)

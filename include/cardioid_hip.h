@@ -177,6 +177,9 @@ typedef struct {
 #define CDH_MODE_FP64_FUSED 1     /* one launch per step, same arithmetic */
 #define CDH_MODE_MIXED_FUSED 2    /* one launch per step; FP32 gates/concentrations and stencil,
                                      FP64 Vm, Ca_i, Na_i */
+#define CDH_MODE_FP64_FMA 3       /* one launch per step, FP64 throughout; the fit polynomials
+                                     use fused multiply-adds (rounding differs from the
+                                     reference's by ulps; Vm parity within 1e-9) */
 
 typedef struct {
    int nx, ny, nz;
@@ -195,8 +198,8 @@ typedef struct {
    int mode;                    /* CDH_MODE_* */
    int gate_tail_rule;          /* 1: the v2 loops' per-group bounds (reference) */
    int sensors;                 /* bit 0: activation times; bit 1: APD90 as well */
-   int variant;                 /* fused kernel occupancy target, waves/SIMD (FP64: 2..4,
-                                   mixed: 3..5); 0 = default */
+   int variant;                 /* fused kernel register target, waves/SIMD (FP64: 2..3,
+                                   mixed: 3..4); 0 = default */
    int check_ranges;            /* Simulate::checkRanges (checkRanges, default on) */
    double vm_min, vm_max;       /* its range (VmMin, VmMax: -150, 100 by default,
                                    initializeSimulate.cc:96-98) */
@@ -290,7 +293,7 @@ int cdh_batch_sync(cdh_batch *b);
 /* steps per captured HIP graph (even; 0 launches each step directly) */
 int cdh_batch_set_graph(cdh_batch *b, int steps);
 /* launch configuration: the kernel instance's register target (waves/SIMD:
- * FP64 2..4, mixed 3..5), the workgroup size (64, 128, 256) and a cap on
+ * FP64 2..3, mixed 3..4), the workgroup size (64, 128, 256) and a cap on
  * the waves per SIMD resident at run time (0: none; applied by reserving
  * LDS per workgroup) */
 int cdh_batch_set_launch(cdh_batch *b, int waves, int block, int occupancy);

@@ -149,7 +149,8 @@ def test_fgr_plugin_matches_oracle_with_remote_cells():
     assert np.array_equal(got, want)
 
 
-def _engine_vs_oracle(anat, celltypes, stims, nsteps, modes=("fp64_unfused", "fp64_fused")):
+def _engine_vs_oracle(anat, celltypes, stims, nsteps,
+                     modes=("fp64_unfused", "fp64_fused", "fp64_fma")):
     import oracle_lib as OL
     ref = OL.OracleSim(anat, celltypes, DT, W.DIFFUSION_SCALE, stims)
     ref.run(nsteps)
@@ -172,9 +173,9 @@ def _engine_vs_oracle(anat, celltypes, stims, nsteps, modes=("fp64_unfused", "fp
             assert same.all(), nm
         out[mode] = (vm, sim.state())
         sim.close()
-    if len(modes) == 2:
-        assert np.array_equal(out[modes[0]][0], out[modes[1]][0])
-        assert np.array_equal(out[modes[0]][1], out[modes[1]][1])
+    # the fused step rounds as the unfused one does, bit for bit
+    assert np.array_equal(out["fp64_unfused"][0], out["fp64_fused"][0])
+    assert np.array_equal(out["fp64_unfused"][1], out["fp64_fused"][1])
     ref.close()
 
 
@@ -212,17 +213,25 @@ def test_engine_apd90_matches_oracle():
     ref.run(n)
     want_act, want_t90 = ref.activation()[1], ref.t90()
     ref.close()
-    sim = hip.Simulation(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims, mode="fp64_fused",
-                         sensors=3)
-    sim.run(n)
-    act, t90 = sim.activation()[1], sim.t90()
-    sim.close()
-    apd = t90 - act
-    say(f"  cells {a.ncell}, repolarised {(t90 > 0).mean():.3f}, APD90 range "
-        f"[{apd[t90 > 0].min():.2f}, {apd[t90 > 0].max():.2f}] ms")
-    assert (t90 > 0).mean() > 0.99
-    assert np.array_equal(act, want_act)
-    assert np.array_equal(t90, want_t90)
+    for mode in ("fp64_fused", "fp64_fma"):
+        sim = hip.Simulation(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims, mode=mode,
+                             sensors=3)
+        sim.run(n)
+        act, t90 = sim.activation()[1], sim.t90()
+        sim.close()
+        apd = t90 - act
+        d90 = np.abs(t90 - want_t90)
+        say(f"  {mode}: cells {a.ncell}, repolarised {(t90 > 0).mean():.3f}, APD90 range "
+            f"[{apd[t90 > 0].min():.2f}, {apd[t90 > 0].max():.2f}] ms, t90 identical "
+            f"{(d90 == 0).mean():.4f}, max diff {d90.max():.3f} ms")
+        assert (t90 > 0).mean() > 0.99
+        assert np.array_equal(act, want_act)
+        if mode == "fp64_fused":
+            assert np.array_equal(t90, want_t90)
+        else:
+            # t90 is a slow crossing (about 0.02 mV per step near 90 % repolarisation);
+            # the FMA rounding may move it by one step
+            assert d90.max() <= DT * 1.0000001
 
 
 @pytest.mark.parametrize("mode", ["fp64_unfused", "fp64_fused", "mixed_fused"])
