@@ -79,7 +79,6 @@ struct EngineArgs {
    const uint16_t *blk_slab;    // per 64-cell tile: slab, or kNoSlab
    SlabDev *slabs;
    double *state;               // FP64: 19 fields in 64-cell blocks
-   const double *A0;            // FP64: centre weight per cell
    float *st32;                 // mixed: 17 fields in 64-cell blocks
    double *st64;                // mixed: Ca_i, Na_i in 64-cell blocks
    const float *Aw;             // 18 off-centre weights in 64-cell blocks

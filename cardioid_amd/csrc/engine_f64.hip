@@ -32,12 +32,21 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 
    // the stencil first (FGRDiffusionOMP::calc order: A0*phi0, then
    // A_k*phi_k for k = 1..18): its neighbour and weight loads are issued with
-   // the state loads at the start, so the reaction's arithmetic covers them
+   // the state loads at the start, so the reaction's arithmetic covers them.
+   // The centre weight is minus the double sum of the 18 float weights in
+   // order (FGRDiffusionOMP.cc:214-219), recomputed here bit for bit rather
+   // than streamed from memory.
    const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
-   double tmp = __builtin_nontemporal_load(a.A0 + i) * V;
+   float w[18];
+   double A0 = 0.0;
 #pragma unroll
-   for (int k = 1; k < 19; k++)
-      tmp += (double)__builtin_nontemporal_load(wp + kBlk * (k - 1)) * box_load(R, b8, S.soff[k]);
+   for (int k = 0; k < 18; k++) {
+      w[k] = __builtin_nontemporal_load(wp + kBlk * k);
+      A0 -= w[k];
+   }
+   double tmp = A0 * V;
+#pragma unroll
+   for (int k = 1; k < 19; k++) tmp += (double)w[k - 1] * box_load(R, b8, S.soff[k]);
    const double dVD = tmp * v.scale;
 
    // states stream once per step (non-temporal); the tile's base address is

@@ -209,7 +209,6 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    ranges_.zero();
    if (mode != CDH_MODE_MIXED_FUSED) {
       track(state_, (size_t)N_STATE * S);
-      track(A0_, S);
    } else {
       track(st32_, (size_t)kMixed32 * S);
       track(st64_, (size_t)2 * S);
@@ -314,7 +313,6 @@ int Engine::add(const SlabHost &h)
    put(bi_.p + c0, bi);
    if (mode_ != CDH_MODE_MIXED_FUSED) {
       put(state_.p + c0 * N_STATE, block(h.state, N_STATE, S, S));
-      put(A0_.p + c0, h.A0);
    } else {
       std::vector<float> s32((size_t)kMixed32 * S);
       std::vector<double> s64((size_t)2 * S);
@@ -391,7 +389,6 @@ EngineArgs Engine::args(int parity) const
    a.blk_slab = blk_slab_.p;
    a.slabs = slabdev_.p;
    a.state = state_.p;
-   a.A0 = A0_.p;
    a.st32 = st32_.p;
    a.st64 = st64_.p;
    a.Aw = Aw_.p;

@@ -140,7 +140,7 @@ class Engine {
    DevBuf<uint32_t> meta_, bi_;
    DevBuf<uint16_t> blk_slab_;
    DevBuf<SlabDev> slabdev_;
-   DevBuf<double> state_, A0_, st64_;
+   DevBuf<double> state_, st64_;
    DevBuf<float> st32_, Aw_;
    DevBuf<double> box_[2];
    DevBuf<double> tact0_, tact_, trec_, vmax_, t90_;

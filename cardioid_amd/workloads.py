@@ -228,3 +228,40 @@ def request_stream(n: int, rate_hz: float = 20.0, seed: int = 4, lo: int = 32, h
         out.append(Request(t, dims, float(rng.uniform(*dur_ms)), rng.uniform(0.0, 1.0, 3),
                            float(rng.uniform(-90.0, 90.0))))
     return out
+
+
+# Niederer et al. (2011) N-version benchmark, as the reference's decks set it
+# up (test/tests/NiedererTest/tools/object.data.proto, mkJobs.sh): a
+# 3 x 7 x 20 mm brick of epicardial cells (cell type 102), fibres along the
+# long (z) axis, the `fibre` conductivity with sigmaLi/sigmaTi below, a box
+# stimulus of -35.71429 uA/uF for 2 ms in the corner cube x, y, z < stimBox.
+NIEDERER_SIGMA_L = 0.0001334177
+NIEDERER_SIGMA_T = 0.0000176062
+NIEDERER_STIMBOX = {0.5: 4, 0.2: 8, 0.1: 16}
+# published activation times (ms) at P1..P8 and the centre, Cardioid r145
+# (test/tests/NiedererTest/tools/table.tex:5-27), keyed by (dx, dt)
+NIEDERER_TABLE = {
+    (0.2, 0.01): [1.24, 35.09, 11.15, 36.43, 39.83, 53.25, 41.58, 54.18, 24.05],
+    (0.2, 0.005): [1.23, 34.72, 11.09, 36.05, 39.63, 52.73, 41.34, 53.65, 23.80],
+    (0.5, 0.01): [1.24, 44.43, 20.79, 47.25, 107.97, 119.63, 109.29, 120.47, 55.38],
+}
+NIEDERER_POINTS = ["P1", "P2", "P3", "P4", "P5", "P6", "P7", "P8", "C"]
+
+
+def niederer(dx: float = 0.2):
+    """(anatomy, stimuli, point cell indices) of the Niederer slab at dx.
+    The points follow the reference's getTimes.py: corners P1..P8 and the
+    centre (nx//2, ny//2, nz//2)."""
+    nx, ny, nz = int(3 / dx), int(7 / dx), int(20 / dx)
+    n = nx * ny * nz
+    f = np.tile(np.array([0.0, 0.0, 1.0]), (n, 1))
+    a = Anatomy(nx, ny, nz, dx, dx, dx, np.arange(n, dtype=np.int64), np.zeros(n, np.uint8),
+                fibre_tensor(f, NIEDERER_SIGMA_L, NIEDERER_SIGMA_T))
+    sb = NIEDERER_STIMBOX[dx]
+    stim = [box_stimulus(xMax=sb, yMax=sb, zMax=sb, nx=nx, ny=ny, nz=nz, vStim=-35.71429,
+                         duration=2.0)]
+    X, Y, Z = nx - 1, ny - 1, nz - 1
+    corners = [(0, 0, 0), (0, 0, Z), (X, 0, 0), (X, 0, Z), (0, Y, 0), (0, Y, Z), (X, Y, 0),
+               (X, Y, Z), (nx // 2, ny // 2, nz // 2)]
+    idx = [x + nx * (y + ny * z) for x, y, z in corners]
+    return a, stim, idx
