@@ -35,6 +35,9 @@ SOURCES = {
     "setup.cpp": ["-x", "hip", "-ffp-contract=off"],
     "fgr_weights.cpp": ["-x", "hip", "-ffp-contract=off"],
     "tt06_host.cpp": ["-x", "hip", "-ffp-contract=off"],
+    "host/reaction.cpp": ["-x", "hip", "-ffp-contract=off"],
+    "host/diffusion.cpp": ["-x", "hip", "-ffp-contract=off"],
+    "host/simulate.hip": ["-ffp-contract=off"],
 }
 
 
@@ -46,7 +49,7 @@ def hipcc():
 
 
 def _compile(src: str, extra):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    obj = os.path.join(BUILD, os.path.splitext(src.replace("/", "_"))[0] + ".o")
     cmd = [hipcc(), *COMMON, *extra, "-fgpu-rdc", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in (r.stderr or "").splitlines() if "packed-fp32-ops" not in l)

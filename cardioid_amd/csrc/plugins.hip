@@ -59,6 +59,9 @@ struct cdh_sensor {
    int evaluated = 0;
    DevBuf<double> tact0, tact, trec;
    DevBuf<uint8_t> active, activated;
+   int apd = 0;
+   DevBuf<uint8_t> apd_phase;
+   DevBuf<double> vmax, t90, vrest;
 };
 
 namespace {
@@ -438,12 +441,12 @@ int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, voi
       a.trec = s->trec.p;
       a.active = s->active.p;
       a.activated = s->activated.p;
-      a.apd = 0;
-      a.apd_phase = nullptr;
-      a.vmax = nullptr;
-      a.t90 = nullptr;
+      a.apd = s->apd;
+      a.apd_phase = s->apd_phase.p;
+      a.vmax = s->vmax.p;
+      a.t90 = s->t90.p;
       hipLaunchKernelGGL(k_sensor_f64, dim3(grid_for(s->n, kBlock)), dim3(kBlock), 0, S(stream), a,
-                         Vm, (const double *)nullptr);
+                         Vm, (const double *)s->vrest.p);
       CDH_LAUNCH_CHECK();
       s->evaluated = 1;
    });
@@ -457,6 +460,35 @@ int cdh_activation_sensor_get(cdh_sensor *s, double *tact0, double *tact, double
       if (tact0) s->tact0.download(tact0, s->n);
       if (tact) s->tact.download(tact, s->n);
       if (trec) s->trec.download(trec, s->n);
+   });
+}
+
+int cdh_activation_sensor_enable_apd(cdh_sensor *s, const double *vrest)
+{
+   return guard([&] {
+      CDH_REQUIRE(s && (vrest || s->n == 0), "cdh_activation_sensor_enable_apd: null argument");
+      CDH_REQUIRE(!s->evaluated, "cdh_activation_sensor_enable_apd: sensor already evaluated");
+      const size_t m = s->n ? s->n : 1;
+      s->apd_phase.alloc(m);
+      s->apd_phase.zero();
+      s->vmax.alloc(m);
+      s->vmax.zero();
+      std::vector<double> neg(m, -1.0);
+      s->t90.alloc(m);
+      s->t90.upload(neg.data(), m);
+      s->vrest.alloc(m);
+      s->vrest.upload(vrest, s->n);
+      s->apd = 1;
+   });
+}
+
+int cdh_activation_sensor_get_t90(cdh_sensor *s, double *t90)
+{
+   return guard([&] {
+      CDH_REQUIRE(s && t90, "cdh_activation_sensor_get_t90: null argument");
+      CDH_REQUIRE(s->apd, "cdh_activation_sensor_get_t90: APD90 not enabled");
+      CDH_HIP(hipDeviceSynchronize());
+      s->t90.download(t90, s->n);
    });
 }
 

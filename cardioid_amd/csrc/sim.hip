@@ -2,12 +2,13 @@
 // (elec/simulationLoop.cc:150-243) with the reaction, diffusion, stimuli
 // and activation sensors device resident.
 //
-// The fused modes are the multi-slab engine holding this one slab.  The
-// unfused FP64 mode keeps the reference's structure, one kernel per stage:
-// iStim = 0 and the box stimuli, TT06Dev_Reaction::calc, the scatter of Vm
-// into the box and FGRDiffusionOMP::calc, then the integrator with the
-// sensors.  Both hold the cells in box order; the input order is kept in
-// the slab's permutation for the host accessors.
+// The fused modes are the multi-slab engine holding this one slab (cells in
+// box order; the input order is kept in the slab's permutation for the host
+// accessors).  The unfused FP64 mode is the reference's own structure, run
+// by the C++ mirror of simulationLoop (host/simulate.hpp) over the drop-in
+// objects: one TT06Dev reaction per cell type in a ReactionManager, the FGR
+// diffusion, box stimuli and activation sensors, all on the device, cells in
+// the caller's (local) order.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -19,16 +20,9 @@
 #include "hip_util.hpp"
 #include "kernels.hpp"
 #include "setup.hpp"
+#include "host/simulate.hpp"
 #include "slab.hpp"
 #include "tt06_f64.hpp"
-
-namespace cdh {
-
-__global__ void k_integrate_sensor_f64(int n, double dt, double *Vm, const double *dVmR,
-                                       const double *dVmD, const double *iStim, SensorArgs s,
-                                       const ParamSet *params, const uint8_t *pid);
-
-}  // namespace cdh
 
 using namespace cdh;
 
@@ -42,271 +36,165 @@ struct cdh_sim {
    // fused modes
    std::unique_ptr<Engine> eng;
    int slab = -1;
-   // unfused mode
-   struct Unfused {
-      int64_t S = 0;
-      BoxGeom geom{};
-      std::vector<int64_t> perm;
-      std::vector<cdh_box_stim> stim;
-      std::vector<int64_t> stimListOff;
-      hipStream_t stream = nullptr;
-      hipEvent_t ev0 = nullptr, ev1 = nullptr;
-      int off[19] = {0};
-      DevBuf<double> coef;
-      DevBuf<ParamSet> params;
-      DevBuf<uint8_t> pid, flags;
-      DevBuf<double> state;              // 19 fields in 64-cell blocks
-      DevBuf<uint32_t> bi;
-      DevBuf<double> A0;
-      DevBuf<float> Aw;                  // 18 fields in 64-cell blocks
-      DevBuf<double> box, Vm, dVmR, dVmD, iStim;
-      DevBuf<int> stimListDev;
-      DevBuf<double> tact0, tact, trec;
-      DevBuf<uint8_t> apdPhase;
-      DevBuf<double> vmax, t90;
-      DevBuf<RangeRec> range;
-      int64_t bytes = 0;
-      ~Unfused()
-      {
-         if (stream) (void)hipStreamSynchronize(stream);
-         if (ev0) (void)hipEventDestroy(ev0);
-         if (ev1) (void)hipEventDestroy(ev1);
-         if (stream) (void)hipStreamDestroy(stream);
-      }
-   };
-   std::unique_ptr<Unfused> u;
+   // unfused mode: the reference's loop over the drop-in objects
+   hipStream_t stream = nullptr;
+   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+   std::unique_ptr<host::Simulate> loopsim;
+   std::vector<host::TT06HipReaction *> reactions;   // owned by the ReactionManager
+   std::vector<std::vector<int>> groups;             // local cells of each reaction
+   int64_t unfusedBytes = 0;
+   BoxGeom unfusedGeom{};
+
+   ~cdh_sim()
+   {
+      if (stream) (void)hipStreamSynchronize(stream);
+      loopsim.reset();
+      if (ev0) (void)hipEventDestroy(ev0);
+      if (ev1) (void)hipEventDestroy(ev1);
+      if (stream) (void)hipStreamDestroy(stream);
+   }
 };
 
 namespace {
 
-constexpr int kBlock = 256;
-
-template <class T>
-void track(cdh_sim::Unfused *u, DevBuf<T> &b, size_t count)
-{
-   b.alloc(count);
-   u->bytes += (int64_t)(count * sizeof(T));
-}
-
-template <class T>
-std::vector<T> to_blocked(const std::vector<T> &soa, int nq, int64_t S)
-{
-   std::vector<T> b(soa.size());
-   for (int q = 0; q < nq; q++)
-      for (int64_t k = 0; k < S; k++) b[blk_index(q, k, nq)] = soa[q * S + k];
-   return b;
-}
-
-// PeriodicPulse::eval (elec/PeriodicPulse.cc:22-28)
-double pulse(const cdh_box_stim &b, double time)
-{
-   double t = time - std::floor(time / b.period) * b.period;
-   if (t >= b.phase && t < b.phase + b.duration) return b.amplitude;
-   return 0.0;
-}
-
-void step_unfused(cdh_sim *s)
-{
-   cdh_sim::Unfused *u = s->u.get();
-   const int n = (int)s->n;
-   const dim3 g(grid_for(n, kBlock)), b(kBlock);
-   // iStim = 0; each active stimulus assigns its value on its cells in order
-   // (Stimulus::stim, elec/Stimulus.hh:21-27; BoxStimulus.cc:53-64)
-   hipLaunchKernelGGL(k_fill_f64, g, b, 0, u->stream, n, 0.0, u->iStim.p);
-   for (size_t k = 0; k < u->stim.size(); k++) {
-      const cdh_box_stim &st = u->stim[k];
-      if (!(s->time > st.t0 && s->time < st.tf)) continue;
-      const double v = pulse(st, s->time);
-      if (v == 0) continue;
-      const int cnt = (int)(u->stimListOff[k + 1] - u->stimListOff[k]);
-      if (cnt == 0) continue;
-      hipLaunchKernelGGL(k_set_list_f64, dim3(grid_for(cnt, kBlock)), b, 0, u->stream, cnt,
-                         u->stimListDev.p + u->stimListOff[k], v, u->iStim.p);
-   }
-   // reaction
-   ReactionArgs r;
-   r.K = s->K;
-   r.coef = u->coef.p;
-   r.params = u->params.p;
-   r.pid = u->pid.p;
-   r.flags = u->flags.p;
-   r.state = u->state.p;
-   r.stride = u->S;
-   r.n = n;
-   r.index = nullptr;
-   r.Vm = u->Vm.p;
-   r.dVm = u->dVmR.p;
-   r.dt = s->p.dt;
-   r.gate_mask = 0xFFFu;
-   hipLaunchKernelGGL(k_tt06_calc_f64<true>, g, b, 0, u->stream, r);
-   // diffusion: scatter Vm into the box, then the stencil
-   hipLaunchKernelGGL(k_scatter_f64, g, b, 0, u->stream, n, u->bi.p, u->Vm.p, u->box.p);
-   StencilArgs d;
-   d.n = n;
-   d.stride = u->S;
-   d.bi = u->bi.p;
-   d.box = u->box.p;
-   d.A0 = u->A0.p;
-   d.Aw = u->Aw.p;
-   for (int k = 0; k < 19; k++) d.off[k] = u->off[k];
-   d.scale = s->p.diffusionScale;
-   d.dVm = u->dVmD.p;
-   hipLaunchKernelGGL(k_fgr_calc_f64<true>, g, b, 0, u->stream, d);
-   if (s->p.check_ranges)
-      hipLaunchKernelGGL(k_check_range_f64, g, b, 0, u->stream, n, u->Vm.p, u->dVmR.p, u->dVmD.p,
-                         s->p.vm_min, s->p.vm_max, s->loop, u->range.p);
-   // integrator and sensors at the time after the step
-   s->time += s->p.dt;
-   ++s->loop;
-   SensorArgs a{};
-   a.n = n;
-   a.enabled = s->p.sensors;
-   a.first_eval = s->loop == 1;
-   a.time = s->time;
-   a.threshold = s->p.threshold;
-   a.tact0 = u->tact0.p;
-   a.tact = u->tact.p;
-   a.trec = u->trec.p;
-   a.apd = (s->p.sensors & 2) ? 1 : 0;
-   a.apd_phase = u->apdPhase.p;
-   a.vmax = u->vmax.p;
-   a.t90 = u->t90.p;
-   hipLaunchKernelGGL(k_integrate_sensor_f64, g, b, 0, u->stream, n, s->p.dt, u->Vm.p, u->dVmR.p,
-                      u->dVmD.p, u->iStim.p, a, u->params.p, u->pid.p);
-   CDH_LAUNCH_CHECK();
-}
-
 void step_n(cdh_sim *s, int nsteps)
 {
-   if (s->eng) {
-      s->eng->run(nsteps);
-      for (int k = 0; k < nsteps; k++) s->time += s->p.dt;
-      s->loop += nsteps;
-      return;
-   }
-   for (int k = 0; k < nsteps; k++) {
-      if (s->u) {
-         step_unfused(s);
-      } else {
-         s->time += s->p.dt;
-         ++s->loop;
-      }
-   }
+   if (s->eng) s->eng->run(nsteps);
+   else if (s->loopsim) host::simulationLoop(*s->loopsim, nsteps);
+   for (int k = 0; k < nsteps; k++) s->time += s->p.dt;
+   s->loop += nsteps;
 }
 
 hipStream_t sim_stream(cdh_sim *s)
 {
    if (s->eng) return s->eng->stream();
-   if (s->u) return s->u->stream;
-   return nullptr;
+   return s->stream;
 }
 
 void sim_sync(cdh_sim *s)
 {
    if (s->eng) s->eng->sync();
-   else if (s->u) CDH_HIP(hipStreamSynchronize(s->u->stream));
-}
-
-// slab-order views of the unfused mode's device arrays
-std::vector<double> unfused_vm(cdh_sim *s)
-{
-   std::vector<double> v(s->n);
-   if (s->n) s->u->Vm.download(v.data(), s->n);
-   return v;
-}
-
-std::vector<double> unfused_state(cdh_sim *s)
-{
-   cdh_sim::Unfused *u = s->u.get();
-   const int64_t S = u->S, n = s->n;
-   std::vector<double> b((size_t)N_STATE * S);
-   u->state.download(b.data(), b.size());
-   std::vector<double> st((size_t)N_STATE * n);
-   for (int q = 0; q < N_STATE; q++)
-      for (int64_t k = 0; k < n; k++) st[q * n + k] = b[blk_index(q, k, N_STATE)];
-   return st;
+   else if (s->stream) CDH_HIP(hipStreamSynchronize(s->stream));
 }
 
 const std::vector<int64_t> &sim_perm(cdh_sim *s)
 {
-   if (s->eng) return s->eng->slab(s->slab).perm;
-   return s->u->perm;
+   return s->eng->slab(s->slab).perm;
 }
 
-void create_unfused(cdh_sim *s, const SlabHost &h, const std::vector<double> &coef)
+// The reference's set-up for this path: reactions per cell type in
+// ascending local order (elec/ReactionManager.cc:184-261), the diffusion
+// (elec/diffusionFactory.cc:38-68), membrane voltage from the reactions
+// (:455-460 of TT06Dev_Reaction.cc), box stimuli with their strict-box cell
+// lists (elec/BoxStimulus.cc:18-34), the activation sensors.
+void create_unfused(cdh_sim *s, const cdh_sim_params *p, int nstim, const cdh_box_stim *stim)
 {
-   auto u = std::make_unique<cdh_sim::Unfused>();
-   const int64_t S = h.S, n = h.n;
-   u->S = S;
-   u->geom = h.geom;
-   u->perm = h.perm;
-   u->stim = h.stim;
-   for (int k = 0; k < 19; k++) u->off[k] = (int)h.off[k];
-   CDH_HIP(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking));
-   CDH_HIP(hipEventCreate(&u->ev0));
-   CDH_HIP(hipEventCreate(&u->ev1));
-   track(u.get(), u->coef, coef.size());
-   u->coef.upload(coef.data(), coef.size());
-   track(u.get(), u->params, h.psets.size());
-   u->params.upload(h.psets.data(), h.psets.size());
-   std::vector<uint8_t> pid(S), fl(S);
-   for (int64_t k = 0; k < S; k++) {
-      pid[k] = (uint8_t)(h.meta[k] & 0xFFu);
-      fl[k] = (uint8_t)((h.meta[k] >> 8) & 0xFFu);
+   CDH_REQUIRE(p->gate_tail_rule,
+               "cdh_sim_create: the unfused mode is the reference's loop; its reactions always "
+               "apply the v2 loop bounds (gate_tail_rule = 1)");
+   const int64_t n = p->ncell;
+   CDH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+   CDH_HIP(hipEventCreate(&s->ev0));
+   CDH_HIP(hipEventCreate(&s->ev1));
+   auto sim = std::make_unique<host::Simulate>(s->stream);
+   sim->dt = p->dt;
+   sim->nLocal = n;
+   sim->Vm.alloc(n);
+   sim->iStim.alloc(n);
+   sim->iStim.zero();
+   sim->dVmReaction.alloc(n);
+   sim->dVmReaction.zero();
+   sim->dVmDiffusion.alloc(n);
+   sim->dVmDiffusion.zero();
+   sim->checkRange = p->check_ranges;
+   sim->vMin = p->vm_min;
+   sim->vMax = p->vm_max;
+   sim->range.alloc(1);
+   sim->range.zero();
+
+   // reactions: one per cell type present, its cells in ascending local order
+   s->groups.assign(p->ncelltype, {});
+   for (int64_t c = 0; c < n; c++) s->groups[p->celltype[c]].push_back((int)c);
+   sim->reaction = std::make_unique<host::ReactionManager>();
+   std::vector<std::vector<int>> kept;
+   for (int t = 0; t < p->ncelltype; t++) {
+      const std::vector<int> &g = s->groups[t];
+      if (g.empty()) continue;
+      cdh_tt06_params tp{};
+      tp.dt = p->dt;
+      tp.tolerance = 1e-4;
+      tp.mod = 1;
+      tp.fastGate = 1;
+      tp.fastNonGate = 0;
+      tp.fit_l = p->fit_l;
+      tp.fit_m = p->fit_m;
+      tp.fit_coef = p->fit_coef;
+      tp.celltype = p->celltypes[t];
+      tp.s_kr = tp.s_cal = tp.s_na = 1.0;
+      if (p->scales) {
+         const double *s0 = p->scales + 3 * g[0];
+         for (int c : g)
+            CDH_REQUIRE(p->scales[3 * c] == s0[0] && p->scales[3 * c + 1] == s0[1] &&
+                            p->scales[3 * c + 2] == s0[2],
+                        "cdh_sim_create: the unfused mode takes one set of drug scales per cell "
+                        "type (one reaction object per type)");
+         tp.s_kr = s0[0];
+         tp.s_cal = s0[1];
+         tp.s_na = s0[2];
+      }
+      auto r = host::reactionFactory("TT06Dev", tp, (int)g.size(), s->stream);
+      s->reactions.push_back(static_cast<host::TT06HipReaction *>(r.get()));
+      sim->reaction->addReaction(std::move(r), g);
+      kept.push_back(g);
    }
-   track(u.get(), u->pid, S);
-   u->pid.upload(pid.data(), S);
-   track(u.get(), u->flags, S);
-   u->flags.upload(fl.data(), S);
-   track(u.get(), u->bi, S);
-   u->bi.upload(h.bi.data(), S);
-   track(u.get(), u->state, (size_t)N_STATE * S);
-   const std::vector<double> st = to_blocked(h.state, N_STATE, S);
-   u->state.upload(st.data(), st.size());
-   track(u.get(), u->A0, S);
-   u->A0.upload(h.A0.data(), S);
-   track(u.get(), u->Aw, (size_t)18 * S);
-   const std::vector<float> aw = to_blocked(h.Aw, 18, S);
-   u->Aw.upload(aw.data(), aw.size());
-   track(u.get(), u->box, h.nbox);
-   u->box.upload(h.box0.data(), h.nbox);
-   std::vector<double> vm(S, 0.0);
-   for (int64_t k = 0; k < n; k++) vm[k] = h.box0[h.bi[k]];
-   track(u.get(), u->Vm, S);
-   u->Vm.upload(vm.data(), S);
-   track(u.get(), u->dVmR, S);
-   u->dVmR.zero();
-   track(u.get(), u->dVmD, S);
-   u->dVmD.zero();
-   track(u.get(), u->iStim, S);
-   u->iStim.zero();
-   std::vector<int> lists;
-   u->stimListOff.assign(1, 0);
-   for (const auto &l : h.stimList) {
-      lists.insert(lists.end(), l.begin(), l.end());
-      u->stimListOff.push_back((int64_t)lists.size());
+   s->groups = kept;
+   sim->reaction->initializeMembraneVoltage(sim->Vm.p);
+
+   // diffusion over the local cells; tissue iff cellType > 0 (FGRUtils.hh:42)
+   std::vector<int> tissueType(n, 1);
+   cdh_fgr_params fp{};
+   fp.nx = p->nx;
+   fp.ny = p->ny;
+   fp.nz = p->nz;
+   fp.dx = p->dx;
+   fp.dy = p->dy;
+   fp.dz = p->dz;
+   fp.diffusionScale = p->diffusionScale;
+   fp.nLocal = n;
+   fp.nRemote = 0;
+   fp.gid = p->gid;
+   fp.cellType = tissueType.data();
+   fp.sigma = p->sigma;
+   sim->diffusion = host::diffusionFactory("FGR", fp, s->stream);
+   s->unfusedGeom = bounding_box(p->nx, p->ny, p->nz, p->gid, n, p->dx, p->dy, p->dz);
+
+   // box stimuli: strict box in grid tuples (elec/BoxStimulus.cc:18-34)
+   for (int q = 0; q < nstim; q++) {
+      CDH_REQUIRE(stim[q].period > 0 && stim[q].duration > 0, "cdh_sim_create: bad pulse");
+      const cdh_box_stim &b = stim[q];
+      std::vector<int> cells;
+      for (int64_t c = 0; c < n; c++) {
+         const int64_t gg = p->gid[c];
+         const int64_t x = gg % p->nx, y = (gg / p->nx) % p->ny, z = gg / ((int64_t)p->nx * p->ny);
+         if (x > b.xMin && x < b.xMax && y > b.yMin && y < b.yMax && z > b.zMin && z < b.zMax)
+            cells.push_back((int)c);
+      }
+      sim->stimulus.push_back(std::make_unique<host::BoxStimulus>(b, cells, s->stream));
    }
-   track(u.get(), u->stimListDev, lists.size() ? lists.size() : 1);
-   u->stimListDev.upload(lists.data(), lists.size());
-   std::vector<double> neg(S, -1.0);
-   track(u.get(), u->range, 1);
-   u->range.zero();
-   track(u.get(), u->tact0, S);
-   u->tact0.upload(neg.data(), S);
-   track(u.get(), u->tact, S);
-   u->tact.upload(neg.data(), S);
-   track(u.get(), u->trec, S);
-   u->trec.upload(neg.data(), S);
-   if (s->p.sensors & 2) {
-      track(u.get(), u->apdPhase, S);
-      u->apdPhase.zero();
-      track(u.get(), u->vmax, S);
-      u->vmax.zero();
-      track(u.get(), u->t90, S);
-      u->t90.upload(neg.data(), S);
+   if (p->sensors) {
+      std::vector<double> vrest;
+      if (p->sensors & 2) {
+         vrest.resize(n);
+         for (int64_t c = 0; c < n; c++) vrest[c] = p->celltypes[p->celltype[c]].Vm0;
+      }
+      sim->sensor.push_back(std::make_unique<host::ActivationSensor>(
+          n, p->threshold, (p->sensors & 2) ? vrest.data() : nullptr, s->stream));
    }
+   const BoxGeom &g = s->unfusedGeom;
+   s->unfusedBytes = n * (int64_t)(N_STATE * 8 + 2 + 4 * 8 + 80 + 4 + 24) +
+                     (int64_t)g.bx * g.by * g.bz * 8;
    CDH_HIP(hipDeviceSynchronize());
-   s->u = std::move(u);
+   s->loopsim = std::move(sim);
 }
 
 }  // namespace
@@ -367,15 +255,15 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
          in.check_ranges = p->check_ranges;
          in.vm_min = p->vm_min;
          in.vm_max = p->vm_max;
-         const SlabHost h = prepare_slab(in, s->K.c9);
          if (p->mode == CDH_MODE_FP64_UNFUSED) {
-            create_unfused(s.get(), h, coef);
+            create_unfused(s.get(), p, nstim, stim);
          } else {
+            const SlabHost h = prepare_slab(in, s->K.c9);
             s->eng = std::make_unique<Engine>(p->mode, p->dt, p->fit_l, p->fit_m, p->fit_coef,
                                               h.S, h.nbox, 1, (int)h.psets.size(), p->sensors);
             LaunchCfg c;
             c.waves = p->variant ? p->variant : 3;
-            c.block = 256;
+            c.block = 64;
             s->eng->set_launch(c);
             s->slab = s->eng->add(h);
             CDH_REQUIRE(s->slab >= 0, "cdh_sim_create: engine arena too small (internal)");
@@ -422,17 +310,17 @@ int cdh_sim_run_timed(cdh_sim *s, int nsteps, double *ms)
          s->loop += nsteps;
          return;
       }
-      if (!s->u) {
+      if (!s->loopsim) {
          step_n(s, nsteps);
          *ms = 0.0;
          return;
       }
-      CDH_HIP(hipEventRecord(s->u->ev0, s->u->stream));
+      CDH_HIP(hipEventRecord(s->ev0, s->stream));
       step_n(s, nsteps);
-      CDH_HIP(hipEventRecord(s->u->ev1, s->u->stream));
-      CDH_HIP(hipEventSynchronize(s->u->ev1));
+      CDH_HIP(hipEventRecord(s->ev1, s->stream));
+      CDH_HIP(hipEventSynchronize(s->ev1));
       float t = 0.0f;
-      CDH_HIP(hipEventElapsedTime(&t, s->u->ev0, s->u->ev1));
+      CDH_HIP(hipEventElapsedTime(&t, s->ev0, s->ev1));
       *ms = t;
    });
 }
@@ -443,7 +331,11 @@ int cdh_sim_get_vm(cdh_sim *s, double *host)
       CDH_REQUIRE(s && (host || s->n == 0), "cdh_sim_get_vm: null argument");
       if (s->n == 0) return;
       sim_sync(s);
-      const std::vector<double> v = s->eng ? s->eng->vm(s->slab) : unfused_vm(s);
+      if (s->loopsim) {
+         s->loopsim->Vm.download(host, s->n);
+         return;
+      }
+      const std::vector<double> v = s->eng->vm(s->slab);
       const std::vector<int64_t> &perm = sim_perm(s);
       for (int64_t k = 0; k < s->n; k++) host[perm[k]] = v[k];
    });
@@ -456,7 +348,17 @@ int cdh_sim_get_state(cdh_sim *s, double *host)
       if (s->n == 0) return;
       sim_sync(s);
       const int64_t n = s->n;
-      const std::vector<double> st = s->eng ? s->eng->state19(s->slab) : unfused_state(s);
+      if (s->loopsim) {
+         for (size_t r = 0; r < s->groups.size(); r++) {
+            const std::vector<int> &g = s->groups[r];
+            std::vector<double> st((size_t)N_STATE * g.size());
+            s->reactions[r]->getState(st.data());
+            for (int q = 0; q < N_STATE; q++)
+               for (size_t k = 0; k < g.size(); k++) host[q * n + g[k]] = st[q * g.size() + k];
+         }
+         return;
+      }
+      const std::vector<double> st = s->eng->state19(s->slab);
       const std::vector<int64_t> &perm = sim_perm(s);
       for (int q = 0; q < N_STATE; q++)
          for (int64_t k = 0; k < n; k++) host[q * n + perm[k]] = st[q * n + k];
@@ -470,30 +372,30 @@ int cdh_sim_set_state(cdh_sim *s, const double *state, const double *Vm)
       if (s->n == 0) return;
       sim_sync(s);
       const int64_t n = s->n;
-      const std::vector<int64_t> &perm = sim_perm(s);
-      std::vector<double> vm = s->eng ? s->eng->vm(s->slab) : unfused_vm(s);
-      if (Vm)
-         for (int64_t k = 0; k < n; k++) vm[k] = Vm[perm[k]];
-      std::vector<double> st;
-      if (state) {
-         st.resize((size_t)N_STATE * n);
-         for (int q = 0; q < N_STATE; q++)
-            for (int64_t k = 0; k < n; k++) st[q * n + k] = state[q * n + perm[k]];
-      }
-      if (s->eng) {
-         if (state) s->eng->set_state19(s->slab, st, vm);
-         if (Vm) s->eng->set_vm(s->slab, vm);
+      if (s->loopsim) {
+         if (state) {
+            for (size_t r = 0; r < s->groups.size(); r++) {
+               const std::vector<int> &g = s->groups[r];
+               std::vector<double> st((size_t)N_STATE * g.size());
+               for (int q = 0; q < N_STATE; q++)
+                  for (size_t k = 0; k < g.size(); k++) st[q * g.size() + k] = state[q * n + g[k]];
+               s->reactions[r]->setState(st.data());
+            }
+         }
+         if (Vm) s->loopsim->Vm.upload(Vm, n);
          return;
       }
-      cdh_sim::Unfused *u = s->u.get();
+      const std::vector<int64_t> &perm = sim_perm(s);
+      std::vector<double> vm = s->eng->vm(s->slab);
+      if (Vm)
+         for (int64_t k = 0; k < n; k++) vm[k] = Vm[perm[k]];
       if (state) {
-         std::vector<double> b((size_t)N_STATE * u->S);
-         u->state.download(b.data(), b.size());
+         std::vector<double> st((size_t)N_STATE * n);
          for (int q = 0; q < N_STATE; q++)
-            for (int64_t k = 0; k < n; k++) b[blk_index(q, k, N_STATE)] = st[q * n + k];
-         u->state.upload(b.data(), b.size());
+            for (int64_t k = 0; k < n; k++) st[q * n + k] = state[q * n + perm[k]];
+         s->eng->set_state19(s->slab, st, vm);
       }
-      if (Vm) u->Vm.upload(vm.data(), n);
+      if (Vm) s->eng->set_vm(s->slab, vm);
    });
 }
 
@@ -504,17 +406,15 @@ int cdh_sim_get_activation(cdh_sim *s, double *tact0, double *tact, double *trec
       if (s->n == 0) return;
       CDH_REQUIRE(s->p.sensors & 1, "cdh_sim_get_activation: the engine was created without sensors");
       sim_sync(s);
+      if (s->loopsim) {
+         s->loopsim->sensor[0]->get(tact0, tact, trec);
+         return;
+      }
       const int64_t n = s->n;
       const std::vector<int64_t> &perm = sim_perm(s);
       std::vector<double> v[3];
       for (auto &x : v) x.resize(n);
-      if (s->eng) {
-         s->eng->activation(s->slab, v[0].data(), v[1].data(), v[2].data());
-      } else {
-         s->u->tact0.download(v[0].data(), n);
-         s->u->tact.download(v[1].data(), n);
-         s->u->trec.download(v[2].data(), n);
-      }
+      s->eng->activation(s->slab, v[0].data(), v[1].data(), v[2].data());
       double *dst[3] = {tact0, tact, trec};
       for (int q = 0; q < 3; q++)
          if (dst[q])
@@ -529,9 +429,12 @@ int cdh_sim_get_t90(cdh_sim *s, double *t90)
       CDH_REQUIRE(s->p.sensors & 2, "cdh_sim_get_t90: the engine was created without APD90");
       if (s->n == 0) return;
       sim_sync(s);
+      if (s->loopsim) {
+         s->loopsim->sensor[0]->t90(t90);
+         return;
+      }
       std::vector<double> v(s->n);
-      if (s->eng) s->eng->t90(s->slab, v.data());
-      else s->u->t90.download(v.data(), s->n);
+      s->eng->t90(s->slab, v.data());
       const std::vector<int64_t> &perm = sim_perm(s);
       for (int64_t k = 0; k < s->n; k++) t90[perm[k]] = v[k];
    });
@@ -543,17 +446,20 @@ int cdh_sim_range_report(cdh_sim *s, int64_t *count, int *loop, int64_t *cell, d
    return guard([&] {
       CDH_REQUIRE(s, "cdh_sim_range_report: null");
       RangeRec r{};
+      int64_t c = -1;
       if (s->eng) {
          r = s->eng->range_report(s->slab);
-      } else if (s->u) {
+         if (r.taken) c = sim_perm(s)[r.cell];
+      } else if (s->loopsim) {
          sim_sync(s);
-         s->u->range.download(&r, 1);
-         s->u->range.zero();
+         s->loopsim->range.download(&r, 1);
+         s->loopsim->range.zero();
+         c = r.cell;
       }
       if (count) *count = (int64_t)r.count;
       if (!r.taken) return;
       if (loop) *loop = r.loop;
-      if (cell) *cell = sim_perm(s)[r.cell];
+      if (cell) *cell = c;
       if (V) *V = r.V;
       if (dVmR) *dVmR = r.dVmR;
       if (dVmD) *dVmD = r.dVmD;
@@ -571,9 +477,9 @@ int cdh_sim_info(const cdh_sim *s, int64_t *device_bytes, int *bx, int *by, int 
    if (s->eng) {
       bytes = s->eng->device_bytes();
       g = s->eng->slab(s->slab).geom;
-   } else if (s->u) {
-      bytes = s->u->bytes;
-      g = s->u->geom;
+   } else if (s->loopsim) {
+      bytes = s->unfusedBytes;
+      g = s->unfusedGeom;
    }
    if (device_bytes) *device_bytes = bytes;
    if (bx) *bx = g.bx;

@@ -94,7 +94,8 @@ EXPORTS = [
     "cdh_fgr_update_remote_voltage", "cdh_fgr_calc", "cdh_fgr_diffusion_scale",
     "cdh_fill", "cdh_set_list", "cdh_integrate",
     "cdh_activation_sensor_create", "cdh_activation_sensor_eval", "cdh_activation_sensor_get",
-    "cdh_activation_sensor_destroy",
+    "cdh_activation_sensor_destroy", "cdh_activation_sensor_enable_apd",
+    "cdh_activation_sensor_get_t90",
     "cdh_sim_create", "cdh_sim_destroy", "cdh_sim_run", "cdh_sim_sync", "cdh_sim_run_timed",
     "cdh_sim_get_vm", "cdh_sim_get_state", "cdh_sim_set_state", "cdh_sim_get_activation",
     "cdh_sim_get_t90", "cdh_sim_range_report",

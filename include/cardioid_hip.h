@@ -158,6 +158,12 @@ int cdh_activation_sensor_create(int64_t n, double threshold, cdh_sensor **out);
 int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, void *stream);
 int cdh_activation_sensor_get(cdh_sensor *s, double *tact0, double *tact, double *trec);
 int cdh_activation_sensor_destroy(cdh_sensor *s);
+/* APD90 on the same sensor (no counterpart in the reference): after the
+ * first up-crossing of the threshold, t90 is the first evaluation time with
+ * Vm < vrest + 0.1*(Vmax - vrest), Vmax the largest Vm since that crossing.
+ * vrest: host array of n rest potentials.  Enable before the first eval. */
+int cdh_activation_sensor_enable_apd(cdh_sensor *s, const double *vrest);
+int cdh_activation_sensor_get_t90(cdh_sensor *s, double *t90);
 
 /* ------------------------------------------------------------------------ */
 /* Monodomain engine: simulationLoop with everything device resident         */
