@@ -31,9 +31,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Gcell-steps/s at 33M cells, FP64 vs mixed-FP32; p50/p99 batched-request latency"
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per tissue cell-step of the fused FP64 step: 19 FP64
-# states read+written (304), Vm read+written (16), 18 float weights + a double
-# centre weight (80)
-FUSED_F64_BYTES = 400
+# states read+written (304), Vm read+written (16), 18 float weights (72); the
+# double centre weight is recomputed from the float ones, not streamed
+FUSED_F64_BYTES = 392
 UNFUSED_F64_BYTES = 480
 # mixed: 17 FP32 states r+w (136), Ca_i/Na_i FP64 r+w (32), Vm r+w (16), 18 FP32 weights (72)
 MIXED_BYTES = 256
