@@ -60,7 +60,7 @@ struct alignas(16) SlabDev {
    int32_t nstim;
    uint32_t stim_on[2];         // bit k: stimulus k is nonzero at time[p]
    int32_t check_range;         // Vm range check on (Simulate::checkRanges)
-   int32_t pad_;
+   float wdelta_scale;          // mixed form: scale of the FP16 weight differences
    double scale;                // diffusionScale
    double threshold;            // activation threshold
    double vm_min, vm_max;       // the check's range
@@ -81,7 +81,9 @@ struct EngineArgs {
    double *state;               // FP64: 19 fields in 64-cell blocks
    float *st32;                 // mixed: 17 fields in 64-cell blocks
    double *st64;                // mixed: Ca_i, Na_i in 64-cell blocks
-   const float *Aw;             // 18 off-centre weights in 64-cell blocks
+   const float *Aw;             // FP64: 18 off-centre weights in 64-cell blocks
+   const float *Aref;           // mixed: per tile, 18 reference weights
+   const uint32_t *Aw16;        // mixed: per cell, FP16 differences, 9 words in 64-cell blocks
    const double *box_cur;       // Vm in the slabs' padded boxes (read)
    double *box_next;            // (written)
    int parity;

@@ -209,11 +209,13 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    ranges_.zero();
    if (mode != CDH_MODE_MIXED_FUSED) {
       track(state_, (size_t)N_STATE * S);
+      track(Aw_, (size_t)18 * S);
    } else {
       track(st32_, (size_t)kMixed32 * S);
       track(st64_, (size_t)2 * S);
+      track(Aref_, (size_t)18 * ntile);
+      track(Aw16_, (size_t)9 * S);
    }
-   track(Aw_, (size_t)18 * S);
    for (int k = 0; k < 2; k++) {
       track(box_[k], cap_box);
       box_[k].zero();
@@ -327,7 +329,12 @@ int Engine::add(const SlabHost &h)
       put(st32_.p + c0 * kMixed32, block(s32, kMixed32, S, S));
       put(st64_.p + c0 * 2, block(s64, 2, S, S));
    }
-   put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
+   if (mode_ != CDH_MODE_MIXED_FUSED) {
+      put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
+   } else {
+      put(Aref_.p + t0 * 18, h.Aref);
+      put(Aw16_.p + c0 * 9, block(h.Aw16, 9, S, S));
+   }
    put(box_[0].p + b0, h.box0);
    put(box_[1].p + b0, h.box0);
    if (sensors_ & 1) {
@@ -352,6 +359,7 @@ int Engine::add(const SlabHost &h)
    d.scale = h.diffusionScale;
    d.threshold = h.threshold;
    d.check_range = h.check_ranges;
+   d.wdelta_scale = h.wdelta_scale;
    d.vm_min = h.vm_min;
    d.vm_max = h.vm_max;
    put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));
@@ -392,6 +400,8 @@ EngineArgs Engine::args(int parity) const
    a.st32 = st32_.p;
    a.st64 = st64_.p;
    a.Aw = Aw_.p;
+   a.Aref = Aref_.p;
+   a.Aw16 = Aw16_.p;
    a.box_cur = box_[parity].p;
    a.box_next = box_[parity ^ 1].p;
    a.parity = parity;

@@ -141,7 +141,8 @@ class Engine {
    DevBuf<uint16_t> blk_slab_;
    DevBuf<SlabDev> slabdev_;
    DevBuf<double> state_, st64_;
-   DevBuf<float> st32_, Aw_;
+   DevBuf<float> st32_, Aw_, Aref_;
+   DevBuf<uint32_t> Aw16_;
    DevBuf<double> box_[2];
    DevBuf<double> tact0_, tact_, trec_, vmax_, t90_;
    DevBuf<uint8_t> apd_phase_;

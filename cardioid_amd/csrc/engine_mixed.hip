@@ -6,7 +6,11 @@
 //    dVm_D = scale * sum_{k=1..18} A_k * (phi_k - phi_0),
 // which is the FGR operator itself (its centre weight is minus the sum of
 // the others, elec/FGRDiffusionOMP.cc:214-219) without the centre weight.
+// The weights A_k are held compactly: a float per tile and neighbour plus
+// an FP16 difference per cell (slab.cpp::compact_weights).
 #pragma clang fp contract(fast)
+
+#include <hip/hip_fp16.h>
 
 #include "engine.hpp"
 #include "tt06_mixed.hpp"
@@ -33,12 +37,21 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
 
    // the stencil first: its neighbour and weight loads are issued with the
    // state loads at the start, so the reaction's arithmetic covers them all
-   const float *wp = a.Aw + (int64_t)blk * (18 * kBlk) + lane;
+   // weight = the tile's reference (wave-uniform, scalar loads) plus the
+   // cell's FP16 difference times the slab's scale, one v_fma_mix each
+   // (exact zero for a zero weight), then the FP32 stencil sum
+   const float *ref = a.Aref + (int64_t)blk * 18;
+   const uint32_t *wp = a.Aw16 + (int64_t)blk * (9 * kBlk) + lane;
+   const float ws = S.wdelta_scale;
    float acc = 0.0f;
 #pragma unroll
-   for (int k = 1; k < 19; k++)
-      acc = __builtin_fmaf(__builtin_nontemporal_load(wp + kBlk * (k - 1)),
-                           (float)(box_load(R, b8, S.soff[k]) - V), acc);
+   for (int p = 0; p < 9; p++) {
+      const __half2 h = __builtin_bit_cast(__half2, __builtin_nontemporal_load(wp + kBlk * p));
+      const float w0 = __builtin_fmaf(__low2float(h), ws, ref[2 * p]);
+      const float w1 = __builtin_fmaf(__high2float(h), ws, ref[2 * p + 1]);
+      acc = __builtin_fmaf(w0, (float)(box_load(R, b8, S.soff[2 * p + 1]) - V), acc);
+      acc = __builtin_fmaf(w1, (float)(box_load(R, b8, S.soff[2 * p + 2]) - V), acc);
+   }
    const double dVD = (double)acc * v.scale;
 
    // states stream once per step (non-temporal); the tile's base addresses

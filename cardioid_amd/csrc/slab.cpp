@@ -2,6 +2,7 @@
 #include "slab.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -32,6 +33,50 @@ void check_input(const SlabInput &in)
    }
    for (int q = 0; q < in.nstim; q++)
       CDH_REQUIRE(in.stim[q].period > 0 && in.stim[q].duration > 0, "slab: bad stimulus pulse");
+}
+
+// Per tile and neighbour a reference weight, per cell its difference from
+// that reference in FP16 after scaling by 2^wexp (chosen so that the slab's
+// largest weight times 2^wexp is at most 2^14: differences stay below FP16's
+// largest finite value, and a difference 2^-28 of the largest weight is
+// still a normal FP16 number).  The reference is the mean of the tile's
+// nonzero weights, rounded to a value whose scaled form FP16 holds exactly:
+// a zero weight (a neighbour outside the tissue, a dropped transverse term)
+// is then stored as the exact negative of the reference and rebuilt as an
+// exact zero by ref + diff*2^-wexp in one fused multiply-add.
+void compact_weights(SlabHost &h)
+{
+   const int64_t S = h.S;
+   double wmax = 0.0;
+   for (float w : h.Aw) wmax = std::max(wmax, (double)std::fabs(w));
+   int wexp = 0;
+   if (wmax > 0.0) wexp = 14 - (int)std::ceil(std::log2(wmax));
+   const double up = std::ldexp(1.0, wexp);
+   h.wdelta_scale = (float)std::ldexp(1.0, -wexp);
+   h.Aref.assign((size_t)h.ntile * 18, 0.0f);
+   h.Aw16.assign((size_t)9 * S, 0u);
+   for (int64_t t = 0; t < h.ntile; t++) {
+      const int64_t c0 = t * kBlk, c1 = std::min<int64_t>(c0 + kBlk, h.n);
+      for (int k = 0; k < 18; k++) {
+         double sum = 0.0;
+         int64_t nz = 0;
+         for (int64_t c = c0; c < c1; c++)
+            if (h.Aw[k * S + c] != 0.0f) {
+               sum += h.Aw[k * S + c];
+               ++nz;
+            }
+         const _Float16 refh = (_Float16)(nz ? sum / (double)nz * up : 0.0);
+         const float ref = (float)((double)(float)refh / up);
+         h.Aref[t * 18 + k] = ref;
+         for (int64_t c = c0; c < c0 + kBlk; c++) {
+            const double d = ((double)h.Aw[k * S + c] - (double)ref) * up;
+            const _Float16 hv = (_Float16)d;
+            uint16_t bits;
+            std::memcpy(&bits, &hv, 2);
+            h.Aw16[(k / 2) * S + c] |= (uint32_t)bits << (16 * (k & 1));
+         }
+      }
+   }
 }
 
 }  // namespace
@@ -147,6 +192,7 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
       for (int q = 0; q < 18; q++) h.Aw[q * S + k] = h.Aw[q * S];
    }
    fgr_offsets(g.bx, g.by, g.bz, h.off);
+   compact_weights(h);
    return h;
 }
 
