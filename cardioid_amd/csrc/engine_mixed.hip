@@ -21,6 +21,23 @@ using namespace mixed;
 
 namespace {
 
+// ref + the low (or high) FP16 half of a word, in one v_fma_mix_f32 with the
+// inline constant 1.0: exact sum, rounded once, so ref + (-ref) is exactly 0
+__device__ __forceinline__ float half_lo_plus(uint32_t packed, float ref)
+{
+   float r;
+   asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(packed), "s"(ref));
+   return r;
+}
+__device__ __forceinline__ float half_hi_plus(uint32_t packed, float ref)
+{
+   float r;
+   asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+       : "=v"(r)
+       : "v"(packed), "s"(ref));
+   return r;
+}
+
 __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S, int sl,
                                            const SlabView &v, int blk, int lane)
 {
@@ -37,22 +54,21 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
 
    // the stencil first: its neighbour and weight loads are issued with the
    // state loads at the start, so the reaction's arithmetic covers them all
-   // weight = the tile's reference (wave-uniform, scalar loads) plus the
-   // cell's FP16 difference times the slab's scale, one v_fma_mix each
-   // (exact zero for a zero weight), then the FP32 stencil sum
+   // weight (in units of 2^-wexp) = the tile's reference (wave-uniform,
+   // scalar loads) plus the cell's FP16 difference, exact zero for a zero
+   // weight; the unit goes into the diffusion scale
    const float *ref = a.Aref + (int64_t)blk * 18;
    const uint32_t *wp = a.Aw16 + (int64_t)blk * (9 * kBlk) + lane;
-   const float ws = S.wdelta_scale;
    float acc = 0.0f;
 #pragma unroll
    for (int p = 0; p < 9; p++) {
-      const __half2 h = __builtin_bit_cast(__half2, __builtin_nontemporal_load(wp + kBlk * p));
-      const float w0 = __builtin_fmaf(__low2float(h), ws, ref[2 * p]);
-      const float w1 = __builtin_fmaf(__high2float(h), ws, ref[2 * p + 1]);
+      const uint32_t h = __builtin_nontemporal_load(wp + kBlk * p);
+      const float w0 = half_lo_plus(h, ref[2 * p]);
+      const float w1 = half_hi_plus(h, ref[2 * p + 1]);
       acc = __builtin_fmaf(w0, (float)(box_load(R, b8, S.soff[2 * p + 1]) - V), acc);
       acc = __builtin_fmaf(w1, (float)(box_load(R, b8, S.soff[2 * p + 2]) - V), acc);
    }
-   const double dVD = (double)acc * v.scale;
+   const double dVD = (double)acc * (v.scale * (double)S.wdelta_scale);
 
    // states stream once per step (non-temporal); the tile's base addresses
    // are wave-uniform

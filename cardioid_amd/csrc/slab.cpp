@@ -66,10 +66,10 @@ void compact_weights(SlabHost &h)
                ++nz;
             }
          const _Float16 refh = (_Float16)(nz ? sum / (double)nz * up : 0.0);
-         const float ref = (float)((double)(float)refh / up);
+         const float ref = (float)refh;                  // in units of 2^-wexp
          h.Aref[t * 18 + k] = ref;
          for (int64_t c = c0; c < c0 + kBlk; c++) {
-            const double d = ((double)h.Aw[k * S + c] - (double)ref) * up;
+            const double d = (double)h.Aw[k * S + c] * up - (double)ref;
             const _Float16 hv = (_Float16)d;
             uint16_t bits;
             std::memcpy(&bits, &hv, 2);

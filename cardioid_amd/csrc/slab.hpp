@@ -48,13 +48,13 @@ struct SlabHost {
    std::vector<double> state;          // [19][S]: rest states, dVK_i = K_i/c9 + Vm0
    std::vector<double> A0;             // S
    std::vector<float> Aw;              // [18][S]
-   // compact weights of the mixed form: per 64-cell tile a float reference
-   // weight per neighbour (the tile's mean), per cell the difference from it
-   // in FP16, scaled by 2^wexp so that the largest fits FP16's range; two
-   // neighbours per 32-bit word ([9][S], low half = even neighbour)
+   // compact weights of the mixed form, in units of 2^-wexp (so that the
+   // largest fits FP16's range): per 64-cell tile a reference weight per
+   // neighbour, per cell the difference from it in FP16; two neighbours per
+   // 32-bit word ([9][S], low half = even neighbour)
    std::vector<float> Aref;            // [ntile][18]
    std::vector<uint32_t> Aw16;         // [9][S]
-   float wdelta_scale = 1.0f;          // 2^-wexp, multiplies the FP16 sums
+   float wdelta_scale = 1.0f;          // 2^-wexp, the unit of Aref and Aw16
    std::vector<double> box0;           // nbox: rest Vm at tissue points, 0 elsewhere
    int64_t off[19] = {0};              // NodeLocation offsets in the box
    std::vector<cdh_box_stim> stim;

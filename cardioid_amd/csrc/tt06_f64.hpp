@@ -40,6 +40,22 @@ __device__ __forceinline__ double fma_vvs(double x, double s, double a)
    return r;
 }
 
+// x*a and x + a with a taken from SGPRs (VOP3, one scalar operand), written
+// out so that no contraction setting fuses them into an FMA whose two
+// coefficients would both need SGPRs
+__device__ __forceinline__ double mul_vs(double x, double a)
+{
+   double r;
+   asm("v_mul_f64 %0, %1, %2" : "=v"(r) : "v"(x), "s"(a));
+   return r;
+}
+__device__ __forceinline__ double add_vs(double x, double a)
+{
+   double r;
+   asm("v_add_f64 %0, %1, %2" : "=v"(r) : "v"(x), "s"(a));
+   return r;
+}
+
 // Horner of a[LO..HI] at x.  FMA = false: separate multiply and add, the
 // reference's rounding (pade.cc:17-36 built without contraction).  FMA =
 // true: fused multiply-adds; the first step stays a multiply and an add,
@@ -50,11 +66,7 @@ template <int LO, int HI, bool FMA = false, class P>
 __device__ __forceinline__ double horner64(P a, double x)
 {
    if constexpr (FMA && HI > LO) {
-      double s;
-      {
-#pragma clang fp contract(off)
-         s = x * a[HI] + a[HI - 1];
-      }
+      double s = add_vs(mul_vs(x, a[HI]), a[HI - 1]);
 #pragma unroll
       for (int j = HI - 2; j >= LO; --j) s = fma_vvs(x, s, a[j]);
       return s;
