@@ -55,6 +55,7 @@ void compact_weights(SlabHost &h)
    h.wdelta_scale = (float)std::ldexp(1.0, -wexp);
    h.Aref.assign((size_t)h.ntile * 18, 0.0f);
    h.Aw16.assign((size_t)9 * S, 0u);
+#pragma omp parallel for schedule(static)
    for (int64_t t = 0; t < h.ntile; t++) {
       const int64_t c0 = t * kBlk, c1 = std::min<int64_t>(c0 + kBlk, h.n);
       for (int k = 0; k < 18; k++) {
@@ -154,6 +155,7 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
    std::vector<uint8_t> stimMask(S, 0);
    h.stim.assign(in.stim, in.stim + in.nstim);
    h.stimList.assign(in.nstim, {});
+#pragma omp parallel for schedule(static)
    for (int64_t k = 0; k < n; k++) {
       const int64_t c = h.perm[k];
       bidx[k] = bidxIn[c];
@@ -169,16 +171,17 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
       const int64_t x = gg % in.nx, y = (gg / in.nx) % in.ny, z = gg / ((int64_t)in.nx * in.ny);
       for (int q = 0; q < in.nstim; q++) {
          const cdh_box_stim &b = in.stim[q];
-         if (x > b.xMin && x < b.xMax && y > b.yMin && y < b.yMax && z > b.zMin && z < b.zMax) {
+         if (x > b.xMin && x < b.xMax && y > b.yMin && y < b.yMax && z > b.zMin && z < b.zMax)
             stimMask[k] |= (uint8_t)(1u << q);
-            h.stimList[q].push_back((int)k);
-         }
       }
       const uint8_t fl = in.gate_tail_rule
                              ? gate_flags(groupPos[c], groupCount[in.celltype[c]])
                              : (uint8_t)0;
       h.meta[k] = meta_pack(pidIn[c], fl, stimMask[k]);
    }
+   for (int64_t k = 0; k < n; k++)
+      for (int q = 0; q < in.nstim; q++)
+         if ((stimMask[k] >> q) & 1u) h.stimList[q].push_back((int)k);
    const CellType &ct0 = in.celltypes[in.celltype[h.perm[0]]];
    for (int64_t k = n; k < S; k++) {
       h.bi[k] = h.bi[0];
