@@ -38,6 +38,7 @@ SOURCES = {
     "host/reaction.cpp": ["-x", "hip", "-ffp-contract=off"],
     "host/diffusion.cpp": ["-x", "hip", "-ffp-contract=off"],
     "host/simulate.hip": ["-ffp-contract=off"],
+    "checkpoint.cpp": ["-x", "hip", "-ffp-contract=off"],
 }
 
 

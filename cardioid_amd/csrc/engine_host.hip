@@ -688,6 +688,22 @@ void Engine::activation(int id, double *tact0, double *tact, double *trec)
    if (trec) download(trec_.p + c0, trec, s.n);
 }
 
+void Engine::set_clock(int id, double time, int loop)
+{
+   check_slab(id);
+   CDH_REQUIRE(loop >= 0, "engine: negative loop");
+   sync();
+   SlabDev d{};
+   download(slabdev_.p + id, &d, 1);
+   d.time[parity_] = time;
+   d.loop[parity_] = loop;
+   slab_stim_eval(d.stim, d.nstim, time, &d.stim_on[parity_], d.stim_val[parity_]);
+   put(slabdev_.p + id, std::vector<SlabDev>{d});
+   sync();
+   slabs_[id].time = time;
+   slabs_[id].loop = loop;
+}
+
 RangeRec Engine::range_report(int id)
 {
    check_slab(id);

@@ -86,6 +86,8 @@ class Engine {
    std::vector<double> state19(int id);                   // [19][n], slab order
    void set_state19(int id, const std::vector<double> &st, const std::vector<double> &vm);
    void set_vm(int id, const std::vector<double> &vm);
+   // restart: the slab's clock (and its stimulus values) at time/loop
+   void set_clock(int id, double time, int loop);
    void activation(int id, double *tact0, double *tact, double *trec);   // slab order
    void t90(int id, double *out);
    // out-of-range record since the last call (cell as a slab-order index);

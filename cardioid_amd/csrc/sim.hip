@@ -30,6 +30,7 @@ struct cdh_sim {
    cdh_sim_params p{};
    int mode = CDH_MODE_FP64_FUSED;
    int64_t n = 0;
+   std::vector<int64_t> gid;             // input order, for checkpoints
    TT06Const K{};
    double time = 0.0;
    int loop = 0;
@@ -228,6 +229,7 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
       s->p = *p;
       s->mode = p->mode;
       s->n = p->ncell;
+      if (p->ncell > 0) s->gid.assign(p->gid, p->gid + p->ncell);
       tt06_constants(&s->K);
       std::vector<double> coef = scaled_fit_coefficients(p->fit_l, p->fit_m, p->fit_coef, p->dt);
       if (p->ncell > 0) {
@@ -463,6 +465,54 @@ int cdh_sim_range_report(cdh_sim *s, int64_t *count, int *loop, int64_t *cell, d
       if (V) *V = r.V;
       if (dVmR) *dVmR = r.dVmR;
       if (dVmD) *dVmD = r.dVmD;
+   });
+}
+
+int cdh_sim_write_checkpoint(cdh_sim *s, const char *dir, int ascii, char *state_path,
+                             int path_len)
+{
+   return guard([&] {
+      CDH_REQUIRE(s && dir, "cdh_sim_write_checkpoint: null argument");
+      std::vector<double> vm(s->n), st((size_t)N_STATE * s->n);
+      if (s->n) {
+         int rc = cdh_sim_get_vm(s, vm.data());
+         if (rc == CDH_OK) rc = cdh_sim_get_state(s, st.data());
+         if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+      }
+      const int rc = cdh_checkpoint_write(dir, "simulate", s->loop, s->time, s->p.nx, s->p.ny,
+                                          s->p.nz, s->n, s->gid.data(), vm.data(), st.data(),
+                                          ascii, state_path, path_len);
+      if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+   });
+}
+
+int cdh_sim_read_checkpoint(cdh_sim *s, const char *state_file)
+{
+   return guard([&] {
+      CDH_REQUIRE(s && state_file, "cdh_sim_read_checkpoint: null argument");
+      // fields the file lacks keep their values
+      std::vector<double> vm(s->n), st((size_t)N_STATE * s->n);
+      if (s->n) {
+         int rc = cdh_sim_get_vm(s, vm.data());
+         if (rc == CDH_OK) rc = cdh_sim_get_state(s, st.data());
+         if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+      }
+      int loop = 0;
+      double time = 0.0;
+      int rc = cdh_checkpoint_read(state_file, s->n, s->gid.data(), vm.data(), st.data(), &loop,
+                                   &time);
+      if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+      if (s->n) {
+         rc = cdh_sim_set_state(s, st.data(), vm.data());
+         if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+      }
+      s->loop = loop;
+      s->time = time;
+      if (s->eng) s->eng->set_clock(s->slab, time, loop);
+      if (s->loopsim) {
+         s->loopsim->time = time;
+         s->loopsim->loop = loop;
+      }
    });
 }
 

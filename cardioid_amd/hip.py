@@ -105,6 +105,8 @@ EXPORTS = [
     "cdh_batch_set_graph", "cdh_batch_set_launch", "cdh_batch_get_launch", "cdh_batch_autotune",
     "cdh_batch_get_vm", "cdh_batch_get_state", "cdh_batch_get_activation", "cdh_batch_get_t90",
     "cdh_batch_range_report", "cdh_batch_slab_info", "cdh_batch_info",
+    "cdh_checkpoint_write", "cdh_checkpoint_read", "cdh_sim_write_checkpoint",
+    "cdh_sim_read_checkpoint",
 ]
 
 ERR_FULL = 5
@@ -194,6 +196,13 @@ def lib():
         L.cdh_batch_get_t90.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_slab_info.argtypes = [C.c_void_p, C.c_int, i64p, dp, ip]
         L.cdh_batch_info.argtypes = [C.c_void_p, i64p, i64p, i64p, ip]
+        L.cdh_checkpoint_write.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_int,
+                                           C.c_int, C.c_int, C.c_int64, i64p, dp, dp, C.c_int,
+                                           C.c_char_p, C.c_int]
+        L.cdh_checkpoint_read.argtypes = [C.c_char_p, C.c_int64, i64p, dp, dp, ip, dp]
+        L.cdh_sim_write_checkpoint.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p,
+                                               C.c_int]
+        L.cdh_sim_read_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
         _lib = L
     return _lib
 
@@ -206,6 +215,40 @@ def check(rc: int, what: str = ""):
 
 def ptr(a, t=dp):
     return a.ctypes.data_as(t)
+
+
+def write_checkpoint(directory: str, loop: int, time: float, box: Sequence[int], gid, Vm, state,
+                     ascii: bool = True, simulate_name: str = "simulate") -> str:
+    """writeCheckpoint (elec/checkpointIO.cc:121-207) of host arrays: gid
+    (n,), Vm (n,), state (19, n) in TT06Dev slot order.  Returns the
+    stateFile name (<directory>/snapshot.<loop>/state) the restart file
+    names."""
+    gid = np.ascontiguousarray(gid, dtype=np.int64)
+    Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+    state = np.ascontiguousarray(state, dtype=np.float64)
+    n = gid.size
+    if Vm.shape != (n,) or state.shape != (NSTATE, n):
+        raise ValueError("write_checkpoint: gid (n,), Vm (n,), state (19, n) expected")
+    out = C.create_string_buffer(4096)
+    check(lib().cdh_checkpoint_write(directory.encode(), simulate_name.encode(), int(loop),
+                                     float(time), int(box[0]), int(box[1]), int(box[2]), n,
+                                     ptr(gid, i64p), ptr(Vm), ptr(state), int(bool(ascii)), out,
+                                     len(out)), "cdh_checkpoint_write")
+    return out.value.decode()
+
+
+def read_checkpoint(state_file: str, gid):
+    """readCheckpoint (elec/checkpointIO.cc:209-259): fields by name,
+    records by gid.  Returns (Vm (n,), state (19, n), loop, time) in the
+    order of `gid`."""
+    gid = np.ascontiguousarray(gid, dtype=np.int64)
+    n = gid.size
+    Vm = np.empty(n)
+    state = np.empty((NSTATE, n))
+    loop, time = C.c_int(), C.c_double()
+    check(lib().cdh_checkpoint_read(state_file.encode(), n, ptr(gid, i64p), ptr(Vm), ptr(state),
+                                    C.byref(loop), C.byref(time)), "cdh_checkpoint_read")
+    return Vm, state, loop.value, time.value
 
 
 def _range_report(call, what):
@@ -321,6 +364,20 @@ class Simulation:
         prints for one of them."""
         return _range_report(lambda *a: lib().cdh_sim_range_report(self.h, *a),
                              "cdh_sim_range_report")
+
+    def write_checkpoint(self, directory: str, ascii: bool = True) -> str:
+        """snapshot.<loop>/state#000000 and restart under `directory`;
+        returns the stateFile name."""
+        out = C.create_string_buffer(4096)
+        check(lib().cdh_sim_write_checkpoint(self.h, directory.encode(), int(bool(ascii)), out,
+                                             len(out)), "cdh_sim_write_checkpoint")
+        return out.value.decode()
+
+    def read_checkpoint(self, state_file: str):
+        """Restart a fresh simulation from a stateFile (states, Vm, loop
+        and time)."""
+        check(lib().cdh_sim_read_checkpoint(self.h, state_file.encode()),
+              "cdh_sim_read_checkpoint")
 
     @property
     def time(self) -> float:

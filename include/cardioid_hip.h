@@ -241,6 +241,33 @@ int cdh_sim_loop(const cdh_sim *s);
 /* bytes of device memory held, and the box dimensions used */
 int cdh_sim_info(const cdh_sim *s, int64_t *device_bytes, int *bx, int *by, int *bz);
 
+/* ------------------------------------------------------------------------ */
+/* Checkpoint / restart files (elec/checkpointIO.cc:121-259)                 */
+/* ------------------------------------------------------------------------ */
+/* writeCheckpoint: <dir>/snapshot.<loop, 12 digits>/state#000000 (a pio
+ * file: `state FILEHEADER` object, blank line, one fixed-length record per
+ * cell: gid, Vm, the 19 TT06Dev fields by name) and .../restart (the
+ * SIMULATE object's loop, time, stateFile).  ascii: FIXRECORDASCII
+ * ("%12llu %21.13e" + " %21.13e" per field), else FIXRECORDBINARY (8-byte
+ * words, exact).  state is [19][n] in TT06Dev slot order.  state_path (may
+ * be NULL) receives the stateFile name. */
+int cdh_checkpoint_write(const char *dir, const char *simulate_name, int loop, double time, int nx,
+                         int ny, int nz, int64_t n, const int64_t *gid, const double *Vm,
+                         const double *state, int ascii, char *state_path, int path_len);
+/* readCheckpoint: fields by name (units converted to the model's), records
+ * by gid (each of the n gids exactly once); Vm and state are in/out: a field
+ * the file lacks is left as passed.  state_file: "<base>", "<base>#" or
+ * "<base>#000000"; files #000001.. of a multi-file state are read in turn. */
+int cdh_checkpoint_read(const char *state_file, int64_t n, const int64_t *gid, double *Vm,
+                        double *state, int *loop, double *time);
+/* the engine's cells through the two calls above (input cell order) */
+int cdh_sim_write_checkpoint(cdh_sim *s, const char *dir, int ascii, char *state_path,
+                             int path_len);
+/* sets the states and Vm from the file (fields it lacks keep their
+ * values); the engine's clock takes the file's loop and time (written with
+ * %f, as the reference restarts from); sensors are not part of a checkpoint */
+int cdh_sim_read_checkpoint(cdh_sim *s, const char *state_file);
+
 
 /* ------------------------------------------------------------------------ */
 /* Batched engine: many slabs advanced together, one launch per step          */
