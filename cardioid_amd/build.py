@@ -39,6 +39,7 @@ SOURCES = {
     "host/diffusion.cpp": ["-x", "hip", "-ffp-contract=off"],
     "host/simulate.hip": ["-ffp-contract=off"],
     "checkpoint.cpp": ["-x", "hip", "-ffp-contract=off"],
+    "setup_device.hip": ["-ffp-contract=off"],
 }
 
 

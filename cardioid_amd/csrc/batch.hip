@@ -46,6 +46,7 @@ SlabInput slab_input(const cdh_slab_desc *d, std::vector<CellType> &cts)
    in.check_ranges = d->check_ranges;
    in.vm_min = d->vm_min;
    in.vm_max = d->vm_max;
+   in.setup_on_host = d->setup_on_host;
    return in;
 }
 

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "setup.hpp"
+#include "setup_device.hpp"
 
 namespace cdh {
 
@@ -329,7 +330,14 @@ int Engine::add(const SlabHost &h)
       put(st32_.p + c0 * kMixed32, block(s32, kMixed32, S, S));
       put(st64_.p + c0 * 2, block(s64, 2, S, S));
    }
-   if (mode_ != CDH_MODE_MIXED_FUSED) {
+   float wdelta_scale = h.wdelta_scale;
+   if (h.device_weights) {
+      const bool f64 = mode_ != CDH_MODE_MIXED_FUSED;
+      device_slab_weights(stream_, h.geom, h.nbox, h.n, h.ntile, h.sig_cells.data(), bi_.p + c0,
+                          (uint32_t)b0, f64 ? Aw_.p + c0 * 18 : nullptr,
+                          f64 ? nullptr : Aref_.p + t0 * 18, f64 ? nullptr : Aw16_.p + c0 * 9,
+                          &wdelta_scale);
+   } else if (mode_ != CDH_MODE_MIXED_FUSED) {
       put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
    } else {
       put(Aref_.p + t0 * 18, h.Aref);
@@ -359,7 +367,7 @@ int Engine::add(const SlabHost &h)
    d.scale = h.diffusionScale;
    d.threshold = h.threshold;
    d.check_range = h.check_ranges;
-   d.wdelta_scale = h.wdelta_scale;
+   d.wdelta_scale = wdelta_scale;
    d.vm_min = h.vm_min;
    d.vm_max = h.vm_max;
    put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));

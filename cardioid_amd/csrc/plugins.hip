@@ -290,20 +290,29 @@ int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
          tissue[ib] = p->cellType[c] > 0 ? 1 : 0;
          std::memcpy(&sig[6 * ib], p->sigma + 6 * c, 6 * sizeof(double));
       }
-      std::vector<double> A0(d->stride, 0.0);
-      std::vector<float> Aw(18 * d->stride, 0.0f);
-      fgr_weights(g, tissue.data(), sig.data(), p->nLocal, bidx.data(), A0.data(), Aw.data(),
-                  d->stride);
       std::vector<uint32_t> b32(ntot);
       for (int64_t c = 0; c < ntot; c++) b32[c] = (uint32_t)bidx[c];
       d->bi.alloc(ntot ? ntot : 1);
       d->bi.upload(b32.data(), ntot);
       d->box.alloc(nbox);
       d->box.zero();
+      // precomputeCoefficients on the device; padding lanes keep zero weights
       d->A0.alloc(d->stride ? d->stride : 1);
-      d->A0.upload(A0.data(), d->stride);
+      d->A0.zero();
       d->Aw.alloc(18 * (d->stride ? d->stride : 1));
-      d->Aw.upload(Aw.data(), Aw.size());
+      d->Aw.zero();
+      if (p->nLocal > 0) {
+         DevBuf<uint8_t> dT(nbox);
+         DevBuf<double> dS(6 * nbox);
+         DevBuf<int64_t> dB(p->nLocal);
+         dT.upload(tissue.data(), nbox);
+         dS.upload(sig.data(), 6 * nbox);
+         dB.upload(bidx.data(), p->nLocal);
+         const int rc = cdh_fgr_weights(g.bx, g.by, g.bz, g.dx, g.dy, g.dz, dT.p, dS.p,
+                                        p->nLocal, dB.p, d->A0.p, d->Aw.p, d->stride, nullptr);
+         if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+         CDH_HIP(hipDeviceSynchronize());
+      }
       int64_t off[19];
       fgr_offsets(g.bx, g.by, g.bz, off);
       StencilArgs &a = d->args;

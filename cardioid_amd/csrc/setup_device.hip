@@ -1,0 +1,280 @@
+// setup_device.hip -- set-up on the device: fibre conductivity from angles
+// and FGR weights from the conductivity and tissue boxes, with the host's
+// rounding (fgr_stencil.hpp; built with -ffp-contract=off, and FP64 division
+// on the device is correctly rounded as on the host).
+//
+//  * k_fibre_sigma: calcConductivityMatrixIBT (elec/FibreConductivity.cc:
+//    27-62) per cell.  From integer angles (the pio anatomy's intType
+//    branch, elec/setConductivity.cc:184-190: angle = i*pi/256) the sines
+//    and cosines come from a host table, so the tensors equal the host's
+//    bit for bit; from double angles the device's sin/cos are used.
+//  * k_fgr_weights: precomputeCoefficients (elec/FGRDiffusionOMP.cc:166-224)
+//    per cell, A0 and the 18 float weights in [k][stride] layout.
+//  * The engine's slab set-up: sigma scattered into the slab's box, weights
+//    in the arena's 64-cell tile layout, and for the mixed form the compact
+//    weights (per-tile FP16-exact reference, per-cell FP16 difference) as
+//    slab.cpp builds them.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cardioid_hip.h"
+#include "fgr_stencil.hpp"
+#include "hip_util.hpp"
+#include "setup_device.hpp"
+#include "tt06_model.hpp"
+
+namespace cdh {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ void ibt(double sp, double cp, double st, double ct, double sL,
+                                    double sT, double *o)
+{
+   const double sp2 = sp * sp, cp2 = cp * cp, st2 = st * st, ct2 = ct * ct;
+   o[0] = (sL * cp2 * st2) + (sT * ((cp2 * ct2) + sp2));
+   o[1] = (sL - sT) * cp * st2 * sp;
+   o[2] = (sL - sT) * ct * cp * st;
+   o[3] = (sL * sp2 * st2) + (sT * (cp2 + (sp2 * ct2)));
+   o[4] = (sL - sT) * sp * ct * st;
+   o[5] = (sL * ct2) + (sT * st2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fibre_sigma(int64_t n, const double *theta,
+                                                        const double *phi, double sL, double sT,
+                                                        double *sigma)
+{
+   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+   if (c >= n) return;
+   ibt(sin(phi[c]), cos(phi[c]), sin(theta[c]), cos(theta[c]), sL, sT, sigma + 6 * c);
+}
+
+// tab[2*(a - lo)] = sin(a pi/256), tab[2*(a - lo) + 1] = cos(a pi/256)
+__global__ __launch_bounds__(kBlock) void k_fibre_sigma_int(int64_t n, const int *theta,
+                                                            const int *phi, int lo, int hi,
+                                                            const double *tab, double sL,
+                                                            double sT, double *sigma,
+                                                            unsigned *bad)
+{
+   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+   if (c >= n) return;
+   const int t = theta[c], p = phi[c];
+   if (t < lo || t > hi || p < lo || p > hi) {
+      atomicAdd(bad, 1u);
+      return;
+   }
+   const double *T = tab + 2 * (t - lo), *P = tab + 2 * (p - lo);
+   ibt(P[0], P[1], T[0], T[1], sL, sT, sigma + 6 * c);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fgr_weights(const uint8_t *tissue, const double *sigma,
+                                                        int64_t n, const int64_t *bidx,
+                                                        fgr::WeightGeom G, double *A0, float *Aw,
+                                                        int64_t stride)
+{
+   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+   if (c >= n) return;
+   float w[19];
+   fgr::cell_weights(tissue, sigma, bidx[c], G, w);
+   A0[c] = fgr::centre_weight(w);
+#pragma unroll
+   for (int k = 1; k < 19; k++) Aw[(k - 1) * stride + c] = w[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter_box(int64_t n, const uint32_t *bi, uint32_t b0,
+                                                        const double *sigc, double *sigbox,
+                                                        uint8_t *tissue)
+{
+   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+   if (k >= n) return;
+   const int64_t ib = (int64_t)(bi[k] - b0);
+   tissue[ib] = 1;
+#pragma unroll
+   for (int q = 0; q < 6; q++) sigbox[6 * ib + q] = sigc[6 * k + q];
+}
+
+// S slab cells (padding cells share cell 0's box point, so they get cell
+// 0's weights, as on the host).  Aw: the arena's tile layout; else W plain
+// [18][S] and the largest |weight| (as float bits) for the compaction.
+__global__ __launch_bounds__(kBlock) void k_engine_weights(const uint8_t *tissue,
+                                                           const double *sigma, int64_t S,
+                                                           const uint32_t *bi, uint32_t b0,
+                                                           fgr::WeightGeom G, float *Aw, float *W,
+                                                           unsigned *wmax)
+{
+   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+   float m = 0.0f;
+   if (c < S) {
+      float w[19];
+      fgr::cell_weights(tissue, sigma, (int64_t)(bi[c] - b0), G, w);
+      if (Aw) {
+#pragma unroll
+         for (int k = 1; k < 19; k++) Aw[blk_index(k - 1, c, 18)] = w[k];
+      } else {
+#pragma unroll
+         for (int k = 1; k < 19; k++) {
+            W[(k - 1) * S + c] = w[k];
+            m = fmaxf(m, fabsf(w[k]));
+         }
+      }
+   }
+   if (wmax) {
+      for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
+      if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(wmax, __float_as_uint(m));
+   }
+}
+
+// one thread per (tile, neighbour pair): the two neighbours' references
+// (mean of the tile's nonzero tissue-cell weights, scaled by up and rounded
+// to FP16, summed in cell order) and the 64 cells' FP16 differences
+__global__ __launch_bounds__(kBlock) void k_compact(const float *W, int64_t S, int64_t n,
+                                                    int64_t ntile, double up, float *Aref,
+                                                    uint32_t *Aw16)
+{
+   const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+   if (g >= ntile * 9) return;
+   const int64_t t = g / 9;
+   const int p = (int)(g % 9);
+   const int64_t c0 = t * kBlk, c1 = c0 + kBlk < n ? c0 + kBlk : n;
+   float ref[2];
+   for (int h = 0; h < 2; h++) {
+      const float *w = W + (int64_t)(2 * p + h) * S;
+      double sum = 0.0;
+      int64_t nz = 0;
+      for (int64_t c = c0; c < c1; c++)
+         if (w[c] != 0.0f) {
+            sum += w[c];
+            ++nz;
+         }
+      ref[h] = (float)fgr::half_rne(nz ? sum / (double)nz * up : 0.0);
+      Aref[t * 18 + 2 * p + h] = ref[h];
+   }
+   for (int64_t c = c0; c < c0 + kBlk; c++) {
+      uint32_t word = 0;
+      for (int h = 0; h < 2; h++) {
+         const double d = (double)W[(int64_t)(2 * p + h) * S + c] * up - (double)ref[h];
+         word |= (uint32_t)fgr::half_bits(fgr::half_rne(d)) << (16 * h);
+      }
+      Aw16[blk_index(p, c, 9)] = word;
+   }
+}
+
+}  // namespace
+
+void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, int64_t n,
+                         int64_t ntile, const double *sig_cells, const uint32_t *bi, uint32_t b0,
+                         float *Aw, float *Aref, uint32_t *Aw16, float *wdelta_scale)
+{
+   const int64_t S = ntile * kBlk;
+   DevBuf<uint8_t> tissue(nbox);
+   DevBuf<double> sigbox(6 * (size_t)nbox), sigc(6 * (size_t)n);
+   CDH_HIP(hipMemsetAsync(tissue.p, 0, nbox, stream));
+   CDH_HIP(hipMemsetAsync(sigbox.p, 0, sizeof(double) * 6 * nbox, stream));
+   CDH_HIP(hipMemcpyAsync(sigc.p, sig_cells, sizeof(double) * 6 * n, hipMemcpyHostToDevice,
+                          stream));
+   hipLaunchKernelGGL(k_scatter_box, dim3(grid_for(n)), dim3(kBlock), 0, stream, n, bi, b0,
+                      sigc.p, sigbox.p, tissue.p);
+   CDH_LAUNCH_CHECK();
+   const fgr::WeightGeom G = fgr::weight_geom(geom);
+   if (Aw) {
+      hipLaunchKernelGGL(k_engine_weights, dim3(grid_for(S)), dim3(kBlock), 0, stream, tissue.p,
+                         sigbox.p, S, bi, b0, G, Aw, (float *)nullptr, (unsigned *)nullptr);
+      CDH_LAUNCH_CHECK();
+      CDH_HIP(hipStreamSynchronize(stream));
+      return;
+   }
+   DevBuf<float> W(18 * (size_t)S);
+   DevBuf<unsigned> wmax(1);
+   CDH_HIP(hipMemsetAsync(wmax.p, 0, sizeof(unsigned), stream));
+   hipLaunchKernelGGL(k_engine_weights, dim3(grid_for(S)), dim3(kBlock), 0, stream, tissue.p,
+                      sigbox.p, S, bi, b0, G, (float *)nullptr, W.p, wmax.p);
+   CDH_LAUNCH_CHECK();
+   unsigned bits = 0;
+   CDH_HIP(hipMemcpyAsync(&bits, wmax.p, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+   CDH_HIP(hipStreamSynchronize(stream));
+   float fmax;
+   std::memcpy(&fmax, &bits, 4);
+   // the exponent rule of slab.cpp's compact_weights
+   const double wm = (double)fmax;
+   const int wexp = wm > 0.0 ? 14 - (int)std::ceil(std::log2(wm)) : 0;
+   *wdelta_scale = (float)std::ldexp(1.0, -wexp);
+   hipLaunchKernelGGL(k_compact, dim3(grid_for(ntile * 9)), dim3(kBlock), 0, stream, W.p, S, n,
+                      ntile, std::ldexp(1.0, wexp), Aref, Aw16);
+   CDH_LAUNCH_CHECK();
+   CDH_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace cdh
+
+using namespace cdh;
+
+extern "C" {
+
+int cdh_fibre_conductivity(int64_t n, const double *theta, const double *phi, double sigmaL,
+                           double sigmaT, double *sigma, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(n >= 0 && (n == 0 || (theta && phi && sigma)),
+                  "cdh_fibre_conductivity: null argument");
+      if (n == 0) return;
+      hipLaunchKernelGGL(k_fibre_sigma, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                         n, theta, phi, sigmaL, sigmaT, sigma);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+int cdh_fibre_conductivity_int(int64_t n, const int *theta, const int *phi, double sigmaL,
+                               double sigmaT, double *sigma, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(n >= 0 && (n == 0 || (theta && phi && sigma)),
+                  "cdh_fibre_conductivity_int: null argument");
+      if (n == 0) return;
+      const int lo = -CDH_FIBRE_ANGLE_MAX, hi = CDH_FIBRE_ANGLE_MAX;
+      std::vector<double> tab(2 * (size_t)(hi - lo + 1));
+      for (int a = lo; a <= hi; a++) {
+         const double ang = a * (M_PI / 256);
+         tab[2 * (a - lo)] = std::sin(ang);
+         tab[2 * (a - lo) + 1] = std::cos(ang);
+      }
+      const hipStream_t s = (hipStream_t)stream;
+      DevBuf<double> dtab(tab.size());
+      DevBuf<unsigned> bad(1);
+      CDH_HIP(hipMemcpyAsync(dtab.p, tab.data(), sizeof(double) * tab.size(),
+                             hipMemcpyHostToDevice, s));
+      CDH_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
+      hipLaunchKernelGGL(k_fibre_sigma_int, dim3(grid_for(n)), dim3(kBlock), 0, s, n, theta, phi,
+                         lo, hi, dtab.p, sigmaL, sigmaT, sigma, bad.p);
+      CDH_LAUNCH_CHECK();
+      unsigned nbad = 0;
+      CDH_HIP(hipMemcpyAsync(&nbad, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+      CDH_HIP(hipStreamSynchronize(s));
+      CDH_REQUIRE(nbad == 0, "cdh_fibre_conductivity_int: an angle outside [-512, 512]");
+   });
+}
+
+int cdh_fgr_weights(int bx, int by, int bz, double dx, double dy, double dz,
+                    const uint8_t *tissue, const double *sigma, int64_t ncell,
+                    const int64_t *blockIndex, double *A0, float *Aw, int64_t stride,
+                    void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(bx >= 3 && by >= 3 && bz >= 3 && dx > 0 && dy > 0 && dz > 0,
+                  "cdh_fgr_weights: bad box");
+      CDH_REQUIRE(ncell >= 0 && stride >= ncell, "cdh_fgr_weights: bad sizes");
+      CDH_REQUIRE(ncell == 0 || (tissue && sigma && blockIndex && A0 && Aw),
+                  "cdh_fgr_weights: null argument");
+      if (ncell == 0) return;
+      BoxGeom g{bx, by, bz, 0, 0, 0, dx, dy, dz};
+      hipLaunchKernelGGL(k_fgr_weights, dim3(grid_for(ncell)), dim3(kBlock), 0,
+                         (hipStream_t)stream, tissue, sigma, ncell, blockIndex,
+                         fgr::weight_geom(g), A0, Aw, stride);
+      CDH_LAUNCH_CHECK();
+   });
+}
+
+}  // extern "C"

@@ -257,6 +257,7 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
          in.check_ranges = p->check_ranges;
          in.vm_min = p->vm_min;
          in.vm_max = p->vm_max;
+         in.setup_on_host = p->setup_on_host;
          if (p->mode == CDH_MODE_FP64_UNFUSED) {
             create_unfused(s.get(), p, nstim, stim);
          } else {

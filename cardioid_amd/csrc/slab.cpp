@@ -9,6 +9,7 @@
 #include <tuple>
 
 #include "engine.hpp"
+#include "fgr_stencil.hpp"
 #include "hip_util.hpp"
 #include "setup.hpp"
 
@@ -66,14 +67,11 @@ void compact_weights(SlabHost &h)
                sum += h.Aw[k * S + c];
                ++nz;
             }
-         const _Float16 refh = (_Float16)(nz ? sum / (double)nz * up : 0.0);
-         const float ref = (float)refh;                  // in units of 2^-wexp
+         const float ref = (float)fgr::half_rne(nz ? sum / (double)nz * up : 0.0);   // units of 2^-wexp
          h.Aref[t * 18 + k] = ref;
          for (int64_t c = c0; c < c0 + kBlk; c++) {
             const double d = (double)h.Aw[k * S + c] * up - (double)ref;
-            const _Float16 hv = (_Float16)d;
-            uint16_t bits;
-            std::memcpy(&bits, &hv, 2);
+            const uint16_t bits = fgr::half_bits(fgr::half_rne(d));
             h.Aw16[(k / 2) * S + c] |= (uint32_t)bits << (16 * (k & 1));
          }
       }
@@ -146,12 +144,19 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
    h.bi.assign(S, 0);
    h.meta.assign(S, 0);
    h.state.assign((size_t)N_STATE * S, 0.0);
-   h.A0.assign(S, 0.0);
-   h.Aw.assign((size_t)18 * S, 0.0f);
    h.box0.assign(h.nbox, 0.0);
-   std::vector<uint8_t> tissue(h.nbox, 0);
-   std::vector<double> sig(6 * h.nbox, 0.0);
+   // host set-up only: tissue and conductivity over the box
+   const bool host = in.setup_on_host != 0;
+   std::vector<uint8_t> tissue(host ? h.nbox : 0, 0);
+   std::vector<double> sig(host ? 6 * h.nbox : 0, 0.0);
    std::vector<int64_t> bidx(n);
+   if (host) {
+      h.A0.assign(S, 0.0);
+      h.Aw.assign((size_t)18 * S, 0.0f);
+   } else {
+      h.device_weights = true;
+      h.sig_cells.resize((size_t)6 * n);
+   }
    std::vector<uint8_t> stimMask(S, 0);
    h.stim.assign(in.stim, in.stim + in.nstim);
    h.stimList.assign(in.nstim, {});
@@ -160,8 +165,12 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
       const int64_t c = h.perm[k];
       bidx[k] = bidxIn[c];
       h.bi[k] = (uint32_t)bidxIn[c];
-      tissue[bidxIn[c]] = 1;
-      std::memcpy(&sig[6 * bidxIn[c]], in.sigma + 6 * c, 6 * sizeof(double));
+      if (host) {
+         tissue[bidxIn[c]] = 1;
+         std::memcpy(&sig[6 * bidxIn[c]], in.sigma + 6 * c, 6 * sizeof(double));
+      } else {
+         std::memcpy(&h.sig_cells[6 * k], in.sigma + 6 * c, 6 * sizeof(double));
+      }
       const CellType &ct = in.celltypes[in.celltype[c]];
       for (int q = 0; q < N_STATE; q++) h.state[q * S + k] = ct.state0[q];
       h.state[S_DVK_I * S + k] = ct.state0[S_DVK_I] / c9 + ct.Vm0;
@@ -189,12 +198,13 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
       for (int q = 0; q < N_STATE; q++) h.state[q * S + k] = ct0.state0[q];
       h.state[S_DVK_I * S + k] = ct0.state0[S_DVK_I] / c9 + ct0.Vm0;
    }
+   fgr_offsets(g.bx, g.by, g.bz, h.off);
+   if (!host) return h;
    fgr_weights(g, tissue.data(), sig.data(), n, bidx.data(), h.A0.data(), h.Aw.data(), S);
    for (int64_t k = n; k < S; k++) {
       h.A0[k] = h.A0[0];
       for (int q = 0; q < 18; q++) h.Aw[q * S + k] = h.Aw[q * S];
    }
-   fgr_offsets(g.bx, g.by, g.bz, h.off);
    compact_weights(h);
    return h;
 }

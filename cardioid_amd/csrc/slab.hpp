@@ -33,6 +33,7 @@ struct SlabInput {
    const cdh_box_stim *stim = nullptr;
    int check_ranges = 1;               // Simulate::checkRanges with [vm_min, vm_max]
    double vm_min = -150.0, vm_max = 100.0;
+   int setup_on_host = 0;              // 0: the weights are built on the device
 };
 
 struct SlabHost {
@@ -55,6 +56,10 @@ struct SlabHost {
    std::vector<float> Aref;            // [ntile][18]
    std::vector<uint32_t> Aw16;         // [9][S]
    float wdelta_scale = 1.0f;          // 2^-wexp, the unit of Aref and Aw16
+   // device set-up: the weights above are left empty and the engine builds
+   // them from the cells' conductivities ([n][6], slab order)
+   bool device_weights = false;
+   std::vector<double> sig_cells;
    std::vector<double> box0;           // nbox: rest Vm at tissue points, 0 elsewhere
    int64_t off[19] = {0};              // NodeLocation offsets in the box
    std::vector<cdh_box_stim> stim;

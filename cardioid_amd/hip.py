@@ -65,7 +65,8 @@ class SimParams(C.Structure):
                 ("celltypes", C.POINTER(CellType)), ("scales", dp), ("fit_l", ip), ("fit_m", ip),
                 ("fit_coef", dp), ("threshold", C.c_double), ("mode", C.c_int),
                 ("gate_tail_rule", C.c_int), ("sensors", C.c_int), ("variant", C.c_int),
-                ("check_ranges", C.c_int), ("vm_min", C.c_double), ("vm_max", C.c_double)]
+                ("check_ranges", C.c_int), ("vm_min", C.c_double), ("vm_max", C.c_double),
+                ("setup_on_host", C.c_int)]
 
 
 class BatchParams(C.Structure):
@@ -81,7 +82,7 @@ class SlabDesc(C.Structure):
                 ("ncelltype", C.c_int), ("celltypes", C.POINTER(CellType)), ("scales", dp),
                 ("threshold", C.c_double), ("gate_tail_rule", C.c_int), ("nstim", C.c_int),
                 ("stim", C.POINTER(BoxStim)), ("check_ranges", C.c_int), ("vm_min", C.c_double),
-                ("vm_max", C.c_double)]
+                ("vm_max", C.c_double), ("setup_on_host", C.c_int)]
 
 
 EXPORTS = [
@@ -106,7 +107,8 @@ EXPORTS = [
     "cdh_batch_get_vm", "cdh_batch_get_state", "cdh_batch_get_activation", "cdh_batch_get_t90",
     "cdh_batch_range_report", "cdh_batch_slab_info", "cdh_batch_info",
     "cdh_checkpoint_write", "cdh_checkpoint_read", "cdh_sim_write_checkpoint",
-    "cdh_sim_read_checkpoint",
+    "cdh_sim_read_checkpoint", "cdh_fibre_conductivity", "cdh_fibre_conductivity_int",
+    "cdh_fgr_weights",
 ]
 
 ERR_FULL = 5
@@ -203,6 +205,13 @@ def lib():
         L.cdh_sim_write_checkpoint.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p,
                                                C.c_int]
         L.cdh_sim_read_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
+        L.cdh_fibre_conductivity.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_double,
+                                             C.c_double, C.c_void_p, C.c_void_p]
+        L.cdh_fibre_conductivity_int.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_double,
+                                                 C.c_double, C.c_void_p, C.c_void_p]
+        L.cdh_fgr_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         _lib = L
     return _lib
 
@@ -278,7 +287,8 @@ class Simulation:
                  stimuli=(), mode: str = "fp64_fused", threshold: float = -40.0,
                  fits: Optional[FitSet] = None, scales: Optional[np.ndarray] = None,
                  gate_tail_rule: bool = True, sensors: int = 1, variant: int = 0,
-                 check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0):
+                 check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0,
+                 setup_on_host: bool = False):
         L = lib()
         self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
         fl, fm, fc = self.fits.packed()
@@ -302,7 +312,7 @@ class Simulation:
                       fit_l=ptr(fl, ip), fit_m=ptr(fm, ip), fit_coef=ptr(fc), threshold=threshold,
                       mode=MODES[mode], gate_tail_rule=int(gate_tail_rule), sensors=int(sensors),
                       variant=variant, check_ranges=int(check_ranges), vm_min=vm_min,
-                      vm_max=vm_max)
+                      vm_max=vm_max, setup_on_host=int(bool(setup_on_host)))
         st = (BoxStim * max(1, len(stimuli)))(*stimuli)
         h = C.c_void_p()
         check(L.cdh_sim_create(C.byref(p), len(stimuli), st, C.byref(h)), "cdh_sim_create")
@@ -401,7 +411,8 @@ class Slab:
 
     def __init__(self, anatomy, celltypes: Sequence[str], diffusion_scale: float, stimuli=(),
                  scales=None, threshold: float = -40.0, gate_tail_rule: bool = True,
-                 check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0):
+                 check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0,
+                 setup_on_host: bool = False):
         cts = (CellType * len(celltypes))(*[standard_celltype(n) for n in celltypes])
         self.ncell = int(anatomy.gid.size)
         gid = np.ascontiguousarray(anatomy.gid, dtype=np.int64)
@@ -421,7 +432,8 @@ class Slab:
                              sigma=ptr(sigma), ncelltype=len(celltypes), celltypes=cts,
                              scales=ptr(sc) if sc is not None else None, threshold=threshold,
                              gate_tail_rule=int(gate_tail_rule), nstim=len(stimuli), stim=st,
-                             check_ranges=int(check_ranges), vm_min=vm_min, vm_max=vm_max)
+                             check_ranges=int(check_ranges), vm_min=vm_min, vm_max=vm_max,
+                             setup_on_host=int(bool(setup_on_host)))
 
     def extent(self):
         """(cells, box points, parameter sets) the slab takes in an arena."""

@@ -143,6 +143,32 @@ int cdh_fgr_calc(cdh_fgr *d, double *dVm, void *stream);
 double cdh_fgr_diffusion_scale(const cdh_fgr *d);
 
 /* ------------------------------------------------------------------------ */
+/* Set-up on the device (all pointers device memory, asynchronous on stream  */
+/* unless stated)                                                            */
+/* ------------------------------------------------------------------------ */
+/* FibreConductivity::compute, calcConductivityMatrixIBT
+ * (elec/FibreConductivity.cc:21-62): sigma[6c..] = {a11 a12 a13 a22 a23 a33}
+ * of fibre angles theta[c], phi[c] (radians), with the device's sin/cos. */
+int cdh_fibre_conductivity(int64_t n, const double *theta, const double *phi, double sigmaL,
+                           double sigmaT, double *sigma, void *stream);
+/* The same for integer angles (angle = i*pi/256, the pio anatomy's integer
+ * fields, elec/setConductivity.cc:184-190), |i| <= CDH_FIBRE_ANGLE_MAX: the
+ * sines and cosines are the host's, so the tensors equal a host
+ * computation bit for bit.  Synchronous; an angle out of range is an error. */
+#define CDH_FIBRE_ANGLE_MAX 512
+int cdh_fibre_conductivity_int(int64_t n, const int *theta, const int *phi, double sigmaL,
+                               double sigmaT, double *sigma, void *stream);
+/* FGRDiffusionOMP::precomputeCoefficients (elec/FGRDiffusionOMP.cc:166-224)
+ * on the device: for ncell cells at box points blockIndex[c] of a
+ * bx*by*bz box (z fastest) holding tissue flags and sigma[6*], A0[c]
+ * (double) and Aw[(k-1)*stride + c] (float, k = 1..18), rounded as the
+ * host build (float weights, one rounding per term, faces in order). */
+int cdh_fgr_weights(int bx, int by, int bz, double dx, double dy, double dz,
+                    const uint8_t *tissue, const double *sigma, int64_t ncell,
+                    const int64_t *blockIndex, double *A0, float *Aw, int64_t stride,
+                    void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Loop pieces                                                               */
 /* ------------------------------------------------------------------------ */
 int cdh_fill(int64_t n, double value, double *x, void *stream);
@@ -209,6 +235,8 @@ typedef struct {
    int check_ranges;            /* Simulate::checkRanges (checkRanges, default on) */
    double vm_min, vm_max;       /* its range (VmMin, VmMax: -150, 100 by default,
                                    initializeSimulate.cc:96-98) */
+   int setup_on_host;           /* 1: FGR weights built on the host; 0 (default): on
+                                   the device, bit for bit the same */
 } cdh_sim_params;
 
 int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim, cdh_sim **out);
@@ -310,6 +338,7 @@ typedef struct {
    const cdh_box_stim *stim;
    int check_ranges;            /* as cdh_sim_params */
    double vm_min, vm_max;
+   int setup_on_host;           /* as cdh_sim_params */
 } cdh_slab_desc;
 
 int cdh_batch_create(const cdh_batch_params *p, cdh_batch **out);

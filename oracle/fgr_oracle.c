@@ -1,6 +1,12 @@
 /*
  * fgr_oracle.c -- CPU restatement of the FGR 19-point diffusion (TEST INFRASTRUCTURE ONLY).
  *
+ * The fibre conductivity follows elec/FibreConductivity.cc:27-62
+ * (calcConductivityMatrixIBT): with s/c the sines and cosines of phi and
+ * theta, a11 = sL c_p^2 s_t^2 + sT (c_p^2 c_t^2 + s_p^2), a22 = sL s_p^2 s_t^2
+ * + sT (c_p^2 + s_p^2 c_t^2), a33 = sL c_t^2 + sT s_t^2, a12 = (sL - sT) c_p
+ * s_t^2 s_p, a13 = (sL - sT) c_t c_p s_t, a23 = (sL - sT) s_p c_t s_t.
+ *
  * Weights follow elec/FGRDiffusionOMP.cc:166-265 and elec/FGRUtils.cc:10-148:
  * per tissue cell and per tissue face neighbour, sigma_bar = (sigma_c +
  * sigma_n)/2, sigmaTimesS = (sigma_bar * S)/V, and
@@ -11,6 +17,7 @@
  *     dVm = scale * (A0*phi0 + sum_{k=1..18} A[k]*phi_k)   in double, in k order.
  */
 #include "oracle.h"
+#include <math.h>
 #include <string.h>
 
 /* NodeLocation order, elec/FGRUtils.hh:19-22 */
@@ -127,6 +134,19 @@ void ora_fgr_weights(int bx, int by, int bz, const uint8_t *tissue,
       for (int k = 0; k < 19; k++) A[19 * c + k] = w[k];
       A0[c] = a0;
    }
+}
+
+void ora_fibre_conductivity(double theta, double phi, double sigmaL, double sigmaT,
+                            double out[6])
+{
+   const double sp = sin(phi), cp = cos(phi), st = sin(theta), ct = cos(theta);
+   const double sp2 = sp * sp, cp2 = cp * cp, st2 = st * st, ct2 = ct * ct;
+   out[0] = (sigmaL * cp2 * st2) + (sigmaT * ((cp2 * ct2) + sp2));
+   out[1] = (sigmaL - sigmaT) * cp * st2 * sp;
+   out[2] = (sigmaL - sigmaT) * ct * cp * st;
+   out[3] = (sigmaL * sp2 * st2) + (sigmaT * (cp2 + (sp2 * ct2)));
+   out[4] = (sigmaL - sigmaT) * sp * ct * st;
+   out[5] = (sigmaL * ct2) + (sigmaT * st2);
 }
 
 void ora_fgr_calc(int64_t ncell, const int64_t *blockIndex, const float *A,

@@ -112,6 +112,10 @@ void ora_fgr_weights(int bx, int by, int bz, const uint8_t *tissue,
                      int64_t ncell, const int64_t *blockIndex,
                      float *A /* 19*ncell, cell-major */, double *A0 /* ncell */);
 void ora_fgr_f2(unsigned iFace, const int tissue[19], double gradPhi[3][19]);
+/* calcConductivityMatrixIBT (elec/FibreConductivity.cc:27-62): out =
+ * {a11, a12, a13, a22, a23, a33} for fibre angles theta, phi (radians) */
+void ora_fibre_conductivity(double theta, double phi, double sigmaL, double sigmaT,
+                            double out[6]);
 void ora_fgr_calc(int64_t ncell, const int64_t *blockIndex, const float *A,
                   const double *A0, const int64_t off[19], const double *VmBlock,
                   double scale, double *dVm);

@@ -17,6 +17,7 @@
 #include "TT06Gates.h"
 #include "TT06NonGates.h"
 #include "FGRUtils.hh"
+#include "FibreConductivity.hh"
 
 namespace {
 const char *FIT_NAMES[33] = {
@@ -148,6 +149,19 @@ void ref_fgr_f2(unsigned iFace, int *tissue, double *gradPhi /* 3*19 */)
    std::memset(g, 0, sizeof(g));
    FGRUtils::f2(iFace, tissue, g);
    std::memcpy(gradPhi, g, sizeof(g));
+}
+
+void ref_fibre_conductivity(double theta, double phi, double sigmaL, double sigmaT,
+                            double *out /* a11 a12 a13 a22 a23 a33 */)
+{
+   FibreConductivityParms p;
+   p.sigmaLi = sigmaL;
+   p.sigmaTi = sigmaT;
+   FibreConductivity fc(p);
+   SymmetricTensor s;
+   fc.compute(theta, phi, s);
+   out[0] = s.a11; out[1] = s.a12; out[2] = s.a13;
+   out[3] = s.a22; out[4] = s.a23; out[5] = s.a33;
 }
 
 } /* extern "C" */

@@ -94,6 +94,7 @@ def oracle():
         L.ora_fgr_weights.argtypes = [C.c_int, C.c_int, C.c_int, u8p, dp, C.c_double, C.c_double,
                                       C.c_double, C.c_int64, i64p, fp, dp]
         L.ora_fgr_f2.argtypes = [C.c_uint, ip, dp]
+        L.ora_fibre_conductivity.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ora_fgr_calc.argtypes = [C.c_int64, i64p, fp, dp, i64p, dp, C.c_double, dp]
         L.ora_sim_create.restype = C.c_void_p
         L.ora_sim_create.argtypes = [C.POINTER(OraSimParams), C.c_int, C.POINTER(OraBoxStim)]
@@ -134,6 +135,7 @@ def ref():
         L.ref_exact_func.restype = C.c_double
         L.ref_exact_func.argtypes = [C.c_int, C.c_double, C.c_int]
         L.ref_fgr_f2.argtypes = [C.c_uint, ip, dp]
+        L.ref_fibre_conductivity.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ref_init()
         _ref = L
     return _ref

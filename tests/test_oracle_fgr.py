@@ -147,3 +147,45 @@ def test_offsets_follow_array3d_layout():
     expect = [0, -dx, -dy, dx, dy, -dz, dz, -dx - dy, dx - dy, dx + dy, -dx + dy,
               -dx - dz, -dy - dz, dx - dz, dy - dz, -dx + dz, -dy + dz, dx + dz, dy + dz]
     assert off.tolist() == expect
+
+
+def _fibre(L, fn, theta, phi, sl, st):
+    out = np.zeros(6)
+    getattr(L, fn)(theta, phi, sl, st, ptr(out))
+    return out
+
+
+@needs_ref
+def test_fibre_conductivity_bitwise_against_reference():
+    """ora_fibre_conductivity equals the reference's FibreConductivity::compute
+    (elec/FibreConductivity.cc, built into oracle/_ref) bit for bit, for the
+    integer-angle grid of pio anatomies and for random double angles."""
+    O, R = oracle(), ref()
+    sl, st = 0.3e-3, 0.0315e-3
+    rng = np.random.default_rng(1)
+    angles = [(i * (np.pi / 256), j * (np.pi / 256)) for i in range(-256, 257, 7)
+              for j in range(-512, 513, 11)]
+    angles += [tuple(rng.uniform(-7, 7, 2)) for _ in range(3000)]
+    for th, ph in angles:
+        a = _fibre(R, "ref_fibre_conductivity", th, ph, sl, st)
+        b = _fibre(O, "ora_fibre_conductivity", th, ph, sl, st)
+        assert np.array_equal(a, b), (th, ph, a, b)
+
+
+def test_fibre_conductivity_closed_form():
+    """sigma = sT I + (sL - sT) f f^T with f = (sin t cos p, sin t sin p, cos t):
+    the fibre direction has eigenvalue sL, its normals sT (so a dropped term,
+    a swapped angle or a wrong sign fails)."""
+    O = oracle()
+    sl, st = 0.1334e-3, 0.0176e-3
+    rng = np.random.default_rng(2)
+    for _ in range(2000):
+        th, ph = rng.uniform(-4, 4, 2)
+        s = _fibre(O, "ora_fibre_conductivity", th, ph, sl, st)
+        f = np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+        M = st * np.eye(3) + (sl - st) * np.outer(f, f)
+        want = np.array([M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2], M[2, 2]])
+        np.testing.assert_allclose(s, want, rtol=0, atol=2e-15 * sl)
+    # theta = 0: fibres along z
+    s = _fibre(O, "ora_fibre_conductivity", 0.0, 0.3, sl, st)
+    np.testing.assert_allclose(s, [st, 0, 0, st, 0, sl], rtol=0, atol=1e-20)
