@@ -138,6 +138,47 @@ def batched(hip, W, steps, tune_steps, graph):
             "achieved_GBps": MIXED_BYTES * ncell * steps / (ms * 1e-3) / 1e9}
 
 
+def shards(hip, W, anat, stims, parts, mode, launch, steps, graph):
+    """The whole anatomy cut into `parts` x-slabs, as `parts` GPUs would hold
+    it: all shares in one engine with their halo planes exchanged after every
+    step (the device-local form of the exchange), and the largest share alone
+    (one GPU's work per step in a `parts`-GPU run, halo aside)."""
+    shares = W.split_x(anat, parts)
+    slabs = [hip.Slab(s.anatomy, W.CELLTYPES_EMI, W.DIFFUSION_SCALE, stims,
+                      remote_gid=s.remote_gid, remote_sigma=s.remote_sigma) for s in shares]
+    ext = [s.extent() for s in slabs]
+    b = hip.Batch(DT, sum(e[0] for e in ext), sum(e[1] for e in ext), slab_capacity=len(slabs),
+                  mode=mode)
+    ids = [b.add(s) for s in slabs]
+    halo = 0
+    for k in range(1, len(ids)):
+        halo += b.link_halo(ids[k], ids[k - 1]) + b.link_halo(ids[k - 1], ids[k])
+    b.set_launch(*launch)
+    b.set_graph(graph)
+    b.run(20)
+    b.sync()
+    ms_all = b.run_timed(steps) / steps
+    b.close()
+    big = max(range(len(slabs)), key=lambda k: slabs[k].ncell)
+    e = ext[big]
+    b = hip.Batch(DT, e[0], e[1], slab_capacity=1, mode=mode)
+    b.add(slabs[big])
+    b.set_launch(*launch)
+    b.set_graph(graph)
+    b.run(20)
+    b.sync()
+    ms_one = b.run_timed(steps) / steps
+    b.close()
+    return {"parts": len(slabs), "mode": mode, "steps": steps,
+            "ms_per_step_all_shares_with_halo_exchange": ms_all,
+            "halo_points_exchanged_per_step": halo,
+            "largest_share_cells": slabs[big].ncell,
+            "largest_share_x_planes": list(shares[big].x_range),
+            "ms_per_step_largest_share_alone": ms_one,
+            "largest_share_Gcell_steps_s": slabs[big].ncell / (ms_one * 1e-3) / 1e9,
+            "largest_share_halo_bytes_per_step": 8 * slabs[big].nremote}
+
+
 def serving(hip, W, nreq, rate, hbm_gb, graph):
     from cardioid_amd import serve
     reqs = W.request_stream(nreq, rate_hz=rate, seed=4)
@@ -169,6 +210,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
+    ap.add_argument("--shard-parts", type=int, default=8)
+    ap.add_argument("--shard-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -273,6 +316,15 @@ def main():
         out["fp64_unfused"] = {"value": u["rate"], "unit": "Gcell-steps/s", "steps": u["steps"],
                                "ms_per_step": u["ms"] / u["steps"],
                                "algorithmic_bytes_per_cell_step": UNFUSED_F64_BYTES}
+    if args.shard_steps > 0 and args.shard_parts > 1:
+        sh = shards(hip, W, anat, stims, args.shard_parts, "fp64_fma", f64["launch"],
+                    args.shard_steps, args.graph)
+        sh["ms_per_step_whole"] = ms_per_step
+        out["shards"] = sh
+        log(f"shards: {sh['parts']} x-slabs with halo exchange "
+            f"{sh['ms_per_step_all_shares_with_halo_exchange']:.3f} ms/step (whole "
+            f"{ms_per_step:.3f}); largest share alone {sh['ms_per_step_largest_share_alone']:.3f} "
+            f"ms/step")
     if args.batched_steps > 0:
         out["batched"] = batched(hip, W, args.batched_steps, args.tune_steps, args.graph)
         log(f"batched: {out['batched']['value']:.3f} Gcell-steps/s")

@@ -47,6 +47,9 @@ SlabInput slab_input(const cdh_slab_desc *d, std::vector<CellType> &cts)
    in.vm_min = d->vm_min;
    in.vm_max = d->vm_max;
    in.setup_on_host = d->setup_on_host;
+   in.nremote = d->nremote;
+   in.remote_gid = d->remote_gid;
+   in.remote_sigma = d->remote_sigma;
    return in;
 }
 
@@ -136,6 +139,14 @@ int cdh_batch_add(cdh_batch *b, const cdh_slab_desc *d, int *slab)
 int cdh_batch_remove(cdh_batch *b, int slab)
 {
    return guard([&] { engine(b).remove(slab); });
+}
+
+int cdh_batch_link_halo(cdh_batch *b, int dst, int src, int64_t *npoints)
+{
+   return guard([&] {
+      const int64_t np = engine(b).link_halo(dst, src);
+      if (npoints) *npoints = np;
+   });
 }
 
 int cdh_batch_run(cdh_batch *b, int nsteps)

@@ -74,6 +74,14 @@ int mixed_slot_source(int q)
 
 namespace {
 
+// HaloExchange within one device: box[dst] = box[src] per (dst, src) pair;
+// destinations are remote points and sources local cells, never the same
+__global__ __launch_bounds__(256) void k_halo(int64_t n, const uint32_t *pairs, double *box)
+{
+   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+   if (i < n) box[pairs[2 * i]] = box[pairs[2 * i + 1]];
+}
+
 using KernelFn = void (*)(EngineArgs);
 
 KernelFn kernel_for(int mode, int waves)
@@ -308,6 +316,10 @@ int Engine::add(const SlabHost &h)
    s.K0.resize(h.n);
    for (int64_t k = 0; k < h.n; k++) s.K0[k] = h.psets[h.meta[k] & 0xFFu].K0;
    s.sensors = sensors_;
+   s.gnx = h.gnx;
+   s.gny = h.gny;
+   s.remote_gid = h.remote_gid;
+   s.remote_bi = h.remote_bi;
 
    put(psets_.p + p0, h.psets);
    put(meta_.p + c0, h.meta);
@@ -334,7 +346,8 @@ int Engine::add(const SlabHost &h)
    if (h.device_weights) {
       const bool f64 = mode_ != CDH_MODE_MIXED_FUSED;
       device_slab_weights(stream_, h.geom, h.nbox, h.n, h.ntile, h.sig_cells.data(), bi_.p + c0,
-                          (uint32_t)b0, f64 ? Aw_.p + c0 * 18 : nullptr,
+                          (uint32_t)b0, (int64_t)h.remote_bi.size(), h.remote_bi.data(),
+                          h.remote_sig.data(), f64 ? Aw_.p + c0 * 18 : nullptr,
                           f64 ? nullptr : Aref_.p + t0 * 18, f64 ? nullptr : Aw16_.p + c0 * 9,
                           &wdelta_scale);
    } else if (mode_ != CDH_MODE_MIXED_FUSED) {
@@ -385,6 +398,11 @@ void Engine::remove(int id)
 {
    check_slab(id);
    retire_staging();
+   const size_t nl = links_.size();
+   links_.erase(std::remove_if(links_.begin(), links_.end(),
+                               [&](const HaloLink &l) { return l.dst == id || l.src == id; }),
+                links_.end());
+   if (links_.size() != nl) upload_halo();
    Slab &s = slabs_[id];
    put(blk_slab_.p + s.tile0, std::vector<uint16_t>(s.ntile, kNoSlab));
    put(&slabdev_.p[id].active, std::vector<int32_t>{0});
@@ -437,6 +455,54 @@ void Engine::launch_step(int parity)
    const dim3 grid(8 * a.groups_per_xcd), blk(cfg_.block);
    hipLaunchKernelGGL(kernel_for(mode_, cfg_.waves), grid, blk, lds_bytes(cfg_), stream_, a);
    CDH_LAUNCH_CHECK();
+   if (nhalo_ > 0) {
+      hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo_ + 255) / 256)), dim3(256), 0, stream_,
+                         nhalo_, halo_.p, box_[parity ^ 1].p);
+      CDH_LAUNCH_CHECK();
+   }
+}
+
+void Engine::upload_halo()
+{
+   std::vector<uint32_t> all;
+   for (const HaloLink &l : links_) all.insert(all.end(), l.pairs.begin(), l.pairs.end());
+   sync();
+   nhalo_ = (int64_t)all.size() / 2;
+   if (halo_.n < all.size()) halo_.alloc(all.size());
+   if (!all.empty()) halo_.upload(all.data(), all.size());
+   ++halo_version_;
+}
+
+int64_t Engine::link_halo(int dst, int src)
+{
+   check_slab(dst);
+   check_slab(src);
+   CDH_REQUIRE(dst != src, "engine: a slab cannot be its own halo source");
+   for (const HaloLink &l : links_)
+      CDH_REQUIRE(!(l.dst == dst && l.src == src), "engine: halo already linked");
+   const Slab &d = slabs_[dst], &s = slabs_[src];
+   CDH_REQUIRE(d.gnx == s.gnx && d.gny == s.gny, "engine: halo slabs on different grids");
+   std::vector<uint8_t> own(s.nbox, 0);
+   for (uint32_t b : s.bi) own[b] = 1;
+   HaloLink l{dst, src, {}};
+   for (size_t r = 0; r < d.remote_gid.size(); r++) {
+      const int64_t bs = box_index(s.geom, s.gnx, s.gny, d.remote_gid[r]);
+      if (bs < 0 || !own[bs]) continue;
+      l.pairs.push_back((uint32_t)(d.box0 + d.remote_bi[r]));
+      l.pairs.push_back((uint32_t)(s.box0 + bs));
+   }
+   CDH_REQUIRE(!l.pairs.empty(), "engine: slab src owns none of slab dst's remote cells");
+   const int64_t np = (int64_t)l.pairs.size() / 2;
+   // the remote points take src's current Vm now, then after every step
+   DevBuf<uint32_t> once(l.pairs.size());
+   once.upload(l.pairs.data(), l.pairs.size());
+   hipLaunchKernelGGL(k_halo, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, stream_, np,
+                      once.p, box_[parity_].p);
+   CDH_LAUNCH_CHECK();
+   sync();
+   links_.push_back(std::move(l));
+   upload_halo();
+   return np;
 }
 
 void Engine::step_eager()
@@ -475,6 +541,7 @@ void Engine::rebuild_graph()
    CDH_HIP(e);
    graph_ntile_ = (int)tiles_.end();
    graph_cfg_ = cfg_;
+   graph_halo_ = halo_version_;
 }
 
 void Engine::set_graph_steps(int k)
@@ -502,7 +569,8 @@ void Engine::run(int nsteps)
    while (nsteps > 0) {
       if (graph_steps_ > 0 && parity_ == 0 && nsteps >= graph_steps_) {
          if (!graph_ || graph_ntile_ != (int)tiles_.end() || graph_cfg_.waves != cfg_.waves ||
-             graph_cfg_.block != cfg_.block || graph_cfg_.occ != cfg_.occ)
+             graph_cfg_.block != cfg_.block || graph_cfg_.occ != cfg_.occ ||
+             graph_halo_ != halo_version_)
             rebuild_graph();
          CDH_HIP(hipGraphLaunch(graph_, stream_));
          for (Slab &s : slabs_)

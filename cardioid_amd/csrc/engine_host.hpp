@@ -53,6 +53,10 @@ class Engine {
       std::vector<int64_t> perm;
       std::vector<uint32_t> bi;        // n: box index within the slab's box
       std::vector<double> K0;          // n: rest K_i (mixed state conversion)
+      int gnx = 0, gny = 0;            // the anatomy grid's x, y sizes (gid layout)
+      // remote cells: tissue box points the slab reads but does not own
+      std::vector<int64_t> remote_gid;
+      std::vector<uint32_t> remote_bi; // box index within the slab's box
       int sensors = 0;
       double time = 0.0;               // host copy of the slab clock
       int loop = 0;
@@ -68,6 +72,11 @@ class Engine {
    // returns the slab id, or -1 if the arena has no room for it
    int add(const SlabHost &h);
    void remove(int id);
+   // Halo exchange: after every step, the remote cells of slab dst that slab
+   // src owns take src's new Vm (the one-GPU form of HaloExchange); also
+   // copies their current Vm now.  Returns the number of points linked.
+   int64_t link_halo(int dst, int src);
+   int64_t halo_points() const { return nhalo_; }
    bool fits(int64_t ntile, int64_t nbox, int64_t nparam) const;
 
    void run(int nsteps);
@@ -105,6 +114,7 @@ class Engine {
  private:
    void step_eager();
    void launch_step(int parity);
+   void upload_halo();
    void rebuild_graph();
    void drop_graph();
    EngineArgs args(int parity) const;
@@ -128,6 +138,15 @@ class Engine {
    hipGraphExec_t graph_ = nullptr;
    int graph_ntile_ = -1;
    LaunchCfg graph_cfg_;
+   int graph_halo_ = -1;
+   struct HaloLink {
+      int dst, src;
+      std::vector<uint32_t> pairs;     // (destination, source) arena box indices
+   };
+   std::vector<HaloLink> links_;
+   DevBuf<uint32_t> halo_;
+   int64_t nhalo_ = 0;
+   int halo_version_ = 0;
    int64_t bytes_ = 0;
    int lds_per_cu_ = 0;
 

@@ -167,6 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const float *W, int64_t S, i
 
 void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, int64_t n,
                          int64_t ntile, const double *sig_cells, const uint32_t *bi, uint32_t b0,
+                         int64_t nremote, const uint32_t *remote_bi, const double *remote_sig,
                          float *Aw, float *Aref, uint32_t *Aw16, float *wdelta_scale)
 {
    const int64_t S = ntile * kBlk;
@@ -179,6 +180,17 @@ void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, 
    hipLaunchKernelGGL(k_scatter_box, dim3(grid_for(n)), dim3(kBlock), 0, stream, n, bi, b0,
                       sigc.p, sigbox.p, tissue.p);
    CDH_LAUNCH_CHECK();
+   DevBuf<uint32_t> rbi(nremote);
+   DevBuf<double> rsig(6 * (size_t)nremote);
+   if (nremote > 0) {
+      CDH_HIP(hipMemcpyAsync(rbi.p, remote_bi, sizeof(uint32_t) * nremote, hipMemcpyHostToDevice,
+                             stream));
+      CDH_HIP(hipMemcpyAsync(rsig.p, remote_sig, sizeof(double) * 6 * nremote,
+                             hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(k_scatter_box, dim3(grid_for(nremote)), dim3(kBlock), 0, stream, nremote,
+                         rbi.p, 0u, rsig.p, sigbox.p, tissue.p);
+      CDH_LAUNCH_CHECK();
+   }
    const fgr::WeightGeom G = fgr::weight_geom(geom);
    if (Aw) {
       hipLaunchKernelGGL(k_engine_weights, dim3(grid_for(S)), dim3(kBlock), 0, stream, tissue.p,

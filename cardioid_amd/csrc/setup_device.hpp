@@ -15,8 +15,11 @@ namespace cdh {
 // cell 0's).  Aw non-null: the 18 float weights in the 64-cell tile layout;
 // else the compact form into Aref ([ntile][18]) and Aw16 (tile layout, 9
 // words) with *wdelta_scale = 2^-wexp, as slab.cpp's compact_weights.
+// Remote cells (nremote, host box indices within the slab's box and their
+// conductivities [nremote][6]) are tissue for the weights.
 void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, int64_t n,
                          int64_t ntile, const double *sig_cells, const uint32_t *bi, uint32_t b0,
+                         int64_t nremote, const uint32_t *remote_bi, const double *remote_sig,
                          float *Aw, float *Aref, uint32_t *Aw16, float *wdelta_scale);
 
 }  // namespace cdh

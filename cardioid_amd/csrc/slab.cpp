@@ -34,6 +34,11 @@ void check_input(const SlabInput &in)
    }
    for (int q = 0; q < in.nstim; q++)
       CDH_REQUIRE(in.stim[q].period > 0 && in.stim[q].duration > 0, "slab: bad stimulus pulse");
+   CDH_REQUIRE(in.nremote >= 0 && (in.nremote == 0 || (in.remote_gid && in.remote_sigma)),
+               "slab: null remote cell arrays");
+   for (int64_t c = 0; c < in.nremote; c++)
+      CDH_REQUIRE(in.remote_gid[c] >= 0 && in.remote_gid[c] < npts,
+                  "slab: remote gid outside the grid");
 }
 
 // Per tile and neighbour a reference weight, per cell its difference from
@@ -197,6 +202,27 @@ SlabHost prepare_slab(const SlabInput &in, double c9)
       h.meta[k] = meta_pack(pidIn[h.perm[0]], kFlagGatesFixed, 0) | kMetaPad;
       for (int q = 0; q < N_STATE; q++) h.state[q * S + k] = ct0.state0[q];
       h.state[S_DVK_I * S + k] = ct0.state0[S_DVK_I] / c9 + ct0.Vm0;
+   }
+   // remote cells: inside the padded box, not local, each once
+   h.gnx = in.nx;
+   h.gny = in.ny;
+   if (in.nremote > 0) {
+      std::vector<uint8_t> taken(h.nbox, 0);
+      for (int64_t k = 0; k < n; k++) taken[h.bi[k]] = 1;
+      h.remote_gid.assign(in.remote_gid, in.remote_gid + in.nremote);
+      h.remote_bi.resize(in.nremote);
+      for (int64_t c = 0; c < in.nremote; c++) {
+         const int64_t rb = box_index(g, in.nx, in.ny, in.remote_gid[c]);
+         CDH_REQUIRE(rb >= 0, "slab: a remote cell lies outside the padded box");
+         CDH_REQUIRE(!taken[rb], "slab: a remote cell is local or listed twice");
+         taken[rb] = 1;
+         h.remote_bi[c] = (uint32_t)rb;
+         if (host) {
+            tissue[rb] = 1;
+            std::memcpy(&sig[6 * rb], in.remote_sigma + 6 * c, 6 * sizeof(double));
+         }
+      }
+      if (!host) h.remote_sig.assign(in.remote_sigma, in.remote_sigma + 6 * in.nremote);
    }
    fgr_offsets(g.bx, g.by, g.bz, h.off);
    if (!host) return h;

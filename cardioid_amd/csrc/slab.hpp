@@ -34,6 +34,11 @@ struct SlabInput {
    int check_ranges = 1;               // Simulate::checkRanges with [vm_min, vm_max]
    double vm_min = -150.0, vm_max = 100.0;
    int setup_on_host = 0;              // 0: the weights are built on the device
+   // remote cells (the reference's nRemote): tissue the slab's stencil reads
+   // but does not update; their Vm comes from the slab that owns them
+   int64_t nremote = 0;
+   const int64_t *remote_gid = nullptr;
+   const double *remote_sigma = nullptr;  // 6*nremote
 };
 
 struct SlabHost {
@@ -60,6 +65,10 @@ struct SlabHost {
    // them from the cells' conductivities ([n][6], slab order)
    bool device_weights = false;
    std::vector<double> sig_cells;
+   int gnx = 0, gny = 0;
+   std::vector<int64_t> remote_gid;
+   std::vector<uint32_t> remote_bi;     // box index within the slab's box
+   std::vector<double> remote_sig;      // [nremote][6]
    std::vector<double> box0;           // nbox: rest Vm at tissue points, 0 elsewhere
    int64_t off[19] = {0};              // NodeLocation offsets in the box
    std::vector<cdh_box_stim> stim;

@@ -82,7 +82,8 @@ class SlabDesc(C.Structure):
                 ("ncelltype", C.c_int), ("celltypes", C.POINTER(CellType)), ("scales", dp),
                 ("threshold", C.c_double), ("gate_tail_rule", C.c_int), ("nstim", C.c_int),
                 ("stim", C.POINTER(BoxStim)), ("check_ranges", C.c_int), ("vm_min", C.c_double),
-                ("vm_max", C.c_double), ("setup_on_host", C.c_int)]
+                ("vm_max", C.c_double), ("setup_on_host", C.c_int), ("nremote", C.c_int64),
+                ("remote_gid", i64p), ("remote_sigma", dp)]
 
 
 EXPORTS = [
@@ -108,7 +109,7 @@ EXPORTS = [
     "cdh_batch_range_report", "cdh_batch_slab_info", "cdh_batch_info",
     "cdh_checkpoint_write", "cdh_checkpoint_read", "cdh_sim_write_checkpoint",
     "cdh_sim_read_checkpoint", "cdh_fibre_conductivity", "cdh_fibre_conductivity_int",
-    "cdh_fgr_weights",
+    "cdh_fgr_weights", "cdh_batch_link_halo",
 ]
 
 ERR_FULL = 5
@@ -185,6 +186,7 @@ def lib():
         L.cdh_slab_extent.argtypes = [C.POINTER(SlabDesc), i64p, i64p, ip]
         L.cdh_batch_add.argtypes = [C.c_void_p, C.POINTER(SlabDesc), ip]
         L.cdh_batch_remove.argtypes = [C.c_void_p, C.c_int]
+        L.cdh_batch_link_halo.argtypes = [C.c_void_p, C.c_int, C.c_int, i64p]
         L.cdh_batch_run.argtypes = [C.c_void_p, C.c_int]
         L.cdh_batch_run_timed.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_sync.argtypes = [C.c_void_p]
@@ -412,7 +414,7 @@ class Slab:
     def __init__(self, anatomy, celltypes: Sequence[str], diffusion_scale: float, stimuli=(),
                  scales=None, threshold: float = -40.0, gate_tail_rule: bool = True,
                  check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0,
-                 setup_on_host: bool = False):
+                 setup_on_host: bool = False, remote_gid=None, remote_sigma=None):
         cts = (CellType * len(celltypes))(*[standard_celltype(n) for n in celltypes])
         self.ncell = int(anatomy.gid.size)
         gid = np.ascontiguousarray(anatomy.gid, dtype=np.int64)
@@ -425,7 +427,13 @@ class Slab:
                 sc = np.tile(sc, self.ncell)
             sc = np.ascontiguousarray(sc.reshape(-1))
         st = (BoxStim * max(1, len(stimuli)))(*stimuli)
-        self._keep = [cts, gid, ctype, sigma, sc, st]
+        rg = np.ascontiguousarray(remote_gid if remote_gid is not None else [], dtype=np.int64)
+        rs = np.ascontiguousarray(remote_sigma if remote_sigma is not None else np.zeros((0, 6)),
+                                  dtype=np.float64).reshape(-1)
+        if rs.size != 6 * rg.size:
+            raise ValueError("remote_sigma must hold 6 values per remote gid")
+        self.nremote = int(rg.size)
+        self._keep = [cts, gid, ctype, sigma, sc, st, rg, rs]
         self.desc = SlabDesc(nx=anatomy.nx, ny=anatomy.ny, nz=anatomy.nz, dx=anatomy.dx,
                              dy=anatomy.dy, dz=anatomy.dz, diffusionScale=diffusion_scale,
                              ncell=self.ncell, gid=ptr(gid, i64p), celltype=ptr(ctype, u8p),
@@ -433,7 +441,9 @@ class Slab:
                              scales=ptr(sc) if sc is not None else None, threshold=threshold,
                              gate_tail_rule=int(gate_tail_rule), nstim=len(stimuli), stim=st,
                              check_ranges=int(check_ranges), vm_min=vm_min, vm_max=vm_max,
-                             setup_on_host=int(bool(setup_on_host)))
+                             setup_on_host=int(bool(setup_on_host)), nremote=rg.size,
+                             remote_gid=ptr(rg, i64p) if rg.size else None,
+                             remote_sigma=ptr(rs) if rs.size else None)
 
     def extent(self):
         """(cells, box points, parameter sets) the slab takes in an arena."""
@@ -490,6 +500,13 @@ class Batch:
     def remove(self, sid: int):
         check(lib().cdh_batch_remove(self.h, sid), "cdh_batch_remove")
         self.ncell.pop(sid, None)
+
+    def link_halo(self, dst: int, src: int) -> int:
+        """After every step, slab dst's remote cells owned by slab src take
+        src's Vm (HaloExchange on one device); returns the points linked."""
+        n = C.c_int64()
+        check(lib().cdh_batch_link_halo(self.h, dst, src, C.byref(n)), "cdh_batch_link_halo")
+        return n.value
 
     def run(self, nsteps: int):
         check(lib().cdh_batch_run(self.h, int(nsteps)), "cdh_batch_run")

@@ -265,3 +265,40 @@ def niederer(dx: float = 0.2):
                (X, Y, Z), (nx // 2, ny // 2, nz // 2)]
     idx = [x + nx * (y + ny * z) for x, y, z in corners]
     return a, stim, idx
+
+
+@dataclass
+class Share:
+    """One part of an anatomy split along x: its own cells, and the remote
+    cells (other parts' tissue inside its padded bounding box) whose Vm it
+    reads from its neighbours each step."""
+    anatomy: Anatomy
+    remote_gid: np.ndarray
+    remote_sigma: np.ndarray      # (nremote, 6)
+    x_range: Tuple[int, int]
+
+
+def split_x(anat: Anatomy, parts: int) -> List[Share]:
+    """Cuts the anatomy into `parts` slabs of whole x planes with about equal
+    cell counts (the slowest box axis, so each part's cells and box are
+    contiguous): the layout one GPU of a `parts`-GPU run would hold."""
+    x, y, z = anat.tuples()
+    counts = np.bincount(x, minlength=anat.nx)
+    cum = np.cumsum(counts)
+    cuts = [0] + [int(np.searchsorted(cum, anat.ncell * p / parts, side="left")) + 1
+                  for p in range(1, parts)] + [anat.nx]
+    cuts = sorted(set(min(max(c, 0), anat.nx) for c in cuts))
+    sig = anat.sigma.reshape(-1, 6)
+    out = []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        own = (x >= a) & (x < b)
+        if not own.any():
+            continue
+        lo = [x[own].min() - 1, y[own].min() - 1, z[own].min() - 1]
+        hi = [x[own].max() + 1, y[own].max() + 1, z[own].max() + 1]
+        near = (~own & (x >= lo[0]) & (x <= hi[0]) & (y >= lo[1]) & (y <= hi[1])
+                & (z >= lo[2]) & (z <= hi[2]))
+        part = Anatomy(anat.nx, anat.ny, anat.nz, anat.dx, anat.dy, anat.dz, anat.gid[own],
+                       anat.celltype[own], sig[own])
+        out.append(Share(part, anat.gid[near], sig[near], (a, b)))
+    return out

@@ -339,6 +339,12 @@ typedef struct {
    int check_ranges;            /* as cdh_sim_params */
    double vm_min, vm_max;
    int setup_on_host;           /* as cdh_sim_params */
+   /* remote cells (Diffusion's nRemote, elec/FGRDiffusionOMP.cc:14-73): tissue
+    * inside the slab's padded box that another slab owns; the stencil weights
+    * count them as tissue and their Vm comes from cdh_batch_link_halo */
+   int64_t nremote;
+   const int64_t *remote_gid;   /* nremote */
+   const double *remote_sigma;  /* 6*nremote */
 } cdh_slab_desc;
 
 int cdh_batch_create(const cdh_batch_params *p, cdh_batch **out);
@@ -348,6 +354,11 @@ int cdh_slab_extent(const cdh_slab_desc *d, int64_t *cells, int64_t *box_points,
 /* adds a slab; CDH_ERR_FULL if the arena has no room for it now */
 int cdh_batch_add(cdh_batch *b, const cdh_slab_desc *d, int *slab);
 int cdh_batch_remove(cdh_batch *b, int slab);
+/* HaloExchange (elec/HaloExchange.hh:197-236) on one device: after every
+ * step the remote cells of slab dst that slab src owns take src's new Vm (and
+ * their current Vm now).  Link every (dst, src) pair of neighbours; a slab's
+ * links end when either slab is removed.  *npoints: points linked. */
+int cdh_batch_link_halo(cdh_batch *b, int dst, int src, int64_t *npoints);
 /* advances every slab by nsteps; asynchronous on the batch's stream */
 int cdh_batch_run(cdh_batch *b, int nsteps);
 int cdh_batch_run_timed(cdh_batch *b, int nsteps, double *ms);

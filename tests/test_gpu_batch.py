@@ -6,6 +6,10 @@
   precisions, with slabs added and removed while others run.
 - Steps replayed from captured HIP graphs equal directly launched steps.
 - The autotuner leaves the simulation where it found it.
+- An anatomy split along x into slabs whose halo planes are exchanged after
+  every step (the one-GPU form of the sharded run) runs as the whole anatomy:
+  bit for bit in FP64; in the mixed form, whose compact weights depend on the
+  tiling, with activation times within 1 dt.
 """
 import sys
 
@@ -196,3 +200,49 @@ def test_arena_full_and_reuse():
     assert b.slab_info(i2)["loop"] == 10
     assert b.info()["nslab"] == 1
     b.close()
+
+
+@pytest.mark.parametrize("mode,parts", [("fp64_fused", 2), ("fp64_fused", 4),
+                                        ("mixed_fused", 3)])
+def test_split_anatomy_with_halo_exchange_equals_whole(mode, parts):
+    a, depth = W.ellipsoid_shell(40, 36, 56, seed=2)
+    stims = W.endocardial_point_stimuli(a, depth, 3, seed=2)
+    say(f"split along x into {parts} with halo exchange vs whole: {mode}, {a.ncell} cells")
+    whole = hip.Slab(a, W.CELLTYPES_EMI, W.DIFFUSION_SCALE, stims, gate_tail_rule=False)
+    shares = W.split_x(a, parts)
+    slabs = [hip.Slab(s.anatomy, W.CELLTYPES_EMI, W.DIFFUSION_SCALE, stims, gate_tail_rule=False,
+                      remote_gid=s.remote_gid, remote_sigma=s.remote_sigma) for s in shares]
+    res = []
+    for group in ([whole], slabs):
+        cells, boxes = capacity(group)
+        b = hip.Batch(DT, cells, boxes, slab_capacity=len(group), mode=mode)
+        ids = [b.add(s) for s in group]
+        linked = 0
+        for k in range(1, len(ids)):
+            linked += b.link_halo(ids[k], ids[k - 1])
+            linked += b.link_halo(ids[k - 1], ids[k])
+        if len(ids) > 1:
+            assert linked == sum(s.nremote for s in group)
+        b.set_graph(10)
+        b.run(450)
+        vm = np.zeros(a.ncell)
+        act = np.zeros(a.ncell)
+        st = np.zeros((19, a.ncell))
+        gids = [a.gid] if len(ids) == 1 else [s.anatomy.gid for s in shares]
+        for sid, g in zip(ids, gids):
+            pos = np.searchsorted(a.gid, g)
+            vm[pos] = b.vm(sid)
+            act[pos] = b.activation(sid)[0]
+            st[:, pos] = b.state(sid)
+        b.close()
+        res.append((vm, act, st))
+    (vm0, act0, st0), (vm1, act1, st1) = res
+    say(f"  activated {(act0 >= 0).sum()} cells; max |dVm| {np.abs(vm1 - vm0).max():.3e} mV")
+    assert (act0 >= 0).sum() > a.ncell // 20
+    if mode == "fp64_fused":
+        assert np.array_equal(vm1, vm0)
+        assert np.array_equal(st1, st0)
+        assert np.array_equal(act1, act0)
+    else:
+        assert np.array_equal(act1 >= 0, act0 >= 0)
+        assert np.abs(act1 - act0).max() <= DT * 1.0000001

@@ -118,3 +118,33 @@ def test_arena_for_budget():
     assert cells % 64 == 0
     per = cells * serve.MIXED_BYTES_PER_CELL + box * serve.BYTES_PER_BOX_POINT
     assert per <= 16 * 2 ** 30 and per > 0.99 * 16 * 2 ** 30 - 64 * 300
+
+
+def test_split_x_partitions_cells_and_names_neighbour_planes():
+    """The x-split of an anatomy (one GPU's share of a sharded run) owns every
+    cell once, and each share's remote cells are exactly the other shares'
+    tissue on the planes next to its own, inside its padded box."""
+    a, _ = W.ellipsoid_shell(30, 26, 40, seed=5)
+    shares = W.split_x(a, 4)
+    assert len(shares) == 4
+    owned = np.concatenate([s.anatomy.gid for s in shares])
+    assert np.array_equal(np.sort(owned), np.sort(a.gid))
+    sizes = [s.anatomy.ncell for s in shares]
+    assert max(sizes) < 1.5 * a.ncell / 4
+    x, y, z = a.tuples()
+    for k, s in enumerate(shares):
+        lo, hi = s.x_range
+        sx, sy, sz = s.anatomy.tuples()
+        rx = s.remote_gid % a.nx
+        assert set(np.unique(rx)) <= {lo - 1, hi}
+        # every cell on the neighbour planes within the padded y/z range is remote
+        box = ((y >= sy.min() - 1) & (y <= sy.max() + 1) & (z >= sz.min() - 1)
+               & (z <= sz.max() + 1) & ((x == lo - 1) | (x == hi))
+               & (x >= sx.min() - 1) & (x <= sx.max() + 1))
+        assert np.array_equal(np.sort(a.gid[box]), np.sort(s.remote_gid))
+        # remote cells belong to the adjacent shares
+        nb = set()
+        for j in (k - 1, k + 1):
+            if 0 <= j < len(shares):
+                nb |= set(shares[j].anatomy.gid.tolist())
+        assert set(s.remote_gid.tolist()) <= nb
