@@ -34,10 +34,11 @@ struct WeightGeom {
    double h[3], hInv[3], vol;
 };
 
-inline WeightGeom weight_geom(const BoxGeom &g)
+// for a box with NodeLocation offsets off[] (any layout)
+inline WeightGeom weight_geom(const BoxGeom &g, const int64_t off[19])
 {
    WeightGeom G;
-   fgr_offsets(g.bx, g.by, g.bz, G.off);
+   for (int k = 0; k < 19; k++) G.off[k] = off[k];
    const int fn[6] = {MZZ, ZMZ, PZZ, ZPZ, ZZM, ZZP};
    for (int f = 0; f < 6; f++) G.face[f] = G.off[fn[f]];
    G.h[0] = g.dx;
@@ -48,6 +49,14 @@ inline WeightGeom weight_geom(const BoxGeom &g)
    G.hInv[2] = 1.0 / g.dz;
    G.vol = G.h[0] * G.h[1] * G.h[2];
    return G;
+}
+
+// for the reference's box layout, z fastest
+inline WeightGeom weight_geom(const BoxGeom &g)
+{
+   int64_t off[19];
+   fgr_offsets(g.bx, g.by, g.bz, off);
+   return weight_geom(g, off);
 }
 
 // one term sigmaTimesS[j] * gradPhi[j][i] * hInv[j] into a float weight

@@ -16,12 +16,16 @@ const int kNodeDelta[19][3] = {
    {1, 0, -1}, {0, 1, -1}, {-1, 0, 1}, {0, -1, 1}, {1, 0, 1}, {0, 1, 1}};
 }  // namespace
 
+void fgr_offsets_strided(int64_t sx, int64_t sy, int64_t sz, int64_t off[19])
+{
+   for (int k = 0; k < 19; k++)
+      off[k] = sx * kNodeDelta[k][0] + sy * kNodeDelta[k][1] + sz * kNodeDelta[k][2];
+}
+
 void fgr_offsets(int bx, int by, int bz, int64_t off[19])
 {
    (void)bx;
-   for (int k = 0; k < 19; k++)
-      off[k] = (int64_t)kNodeDelta[k][2] +
-               (int64_t)bz * ((int64_t)kNodeDelta[k][1] + (int64_t)by * kNodeDelta[k][0]);
+   fgr_offsets_strided((int64_t)by * bz, bz, 1, off);
 }
 
 void fgr_weights(const BoxGeom &g, const uint8_t *tissue, const double *sigma, int64_t ncell,

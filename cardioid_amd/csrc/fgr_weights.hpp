@@ -14,6 +14,8 @@ struct BoxGeom {
 
 // NodeLocation offsets (elec/FGRUtils.hh:19-22) in the box.
 void fgr_offsets(int bx, int by, int bz, int64_t off[19]);
+// The same for a box whose x, y and z neighbours lie sx, sy and sz apart.
+void fgr_offsets_strided(int64_t sx, int64_t sy, int64_t sz, int64_t off[19]);
 
 // Weights for ncell cells at box indices blockIndex[]: A0[c] (double) and
 // Aw[(k-1)*stride + c] (float, k = 1..18).  tissue[] and sigma[6*] are over

@@ -37,10 +37,12 @@ struct StencilArgs {
    const uint32_t *bi;          // box index per cell
    const double *box;           // Vm in the padded box
    const double *A0;            // centre weight (double)
-   const float *Aw;             // [18][stride] off-centre weights, NodeLocation order 1..18
+   const float *Aw;             // off-centre weights, NodeLocation order 1..18: [18][stride],
+                                // or 64-cell tiles (blk_index) for the BLK kernel
    int off[19];                 // NodeLocation offsets in the box
    double scale;                // diffusionScale
    double *dVm;
+   int groups_per_xcd;          // 256-cell blocks per XCD (grid = 8 * groups_per_xcd)
 };
 
 // Activation sensors, evaluated after the step at the new time.

@@ -69,16 +69,29 @@ __global__ __launch_bounds__(256) void k_scatter_f64(int n, const uint32_t *__re
    if (i < n) box[bi[i]] = src[i];
 }
 
+// Blocks are dealt round-robin to the 8 XCDs; each XCD takes a contiguous
+// range of cells, so that its L2 holds the box rows and planes they read.
 template <bool BLK>
 __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
 {
-   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   const int xcd = blockIdx.x & 7;
+   const int64_t i =
+       ((int64_t)xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * blockDim.x + threadIdx.x;
    if (i >= a.n) return;
    const double *phi = a.box + a.bi[i];
-   double tmp = a.A0[i] * phi[0];
+   // the weights stream once per step; the centre weight is minus their sum
+   // in order, in double (FGRDiffusionOMP.cc:214-219), recomputed bit for bit
+   float w[18];
+   double A0 = 0.0;
 #pragma unroll
-   for (int k = 1; k < 19; k++) tmp += (double)a.Aw[sidx<BLK>(k - 1, i, 18, a.stride)] * phi[a.off[k]];
-   a.dVm[i] = tmp * a.scale;
+   for (int k = 0; k < 18; k++) {
+      w[k] = __builtin_nontemporal_load(a.Aw + sidx<BLK>(k, i, 18, a.stride));
+      A0 -= w[k];
+   }
+   double tmp = A0 * phi[0];
+#pragma unroll
+   for (int k = 1; k < 19; k++) tmp += (double)w[k - 1] * phi[a.off[k]];
+   __builtin_nontemporal_store(tmp * a.scale, a.dVm + i);
 }
 template __global__ void k_fgr_calc_f64<false>(StencilArgs);
 template __global__ void k_fgr_calc_f64<true>(StencilArgs);

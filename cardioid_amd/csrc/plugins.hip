@@ -10,6 +10,7 @@
 #include "../../include/cardioid_hip.h"
 #include "fgr_weights.hpp"
 #include "hip_util.hpp"
+#include "setup_device.hpp"
 #include "kernels.hpp"
 #include "setup.hpp"
 #include "tt06_f64.hpp"
@@ -280,11 +281,24 @@ int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
       const BoxGeom &g = d->geom;
       const size_t nbox = (size_t)g.bx * g.by * g.bz;
       CDH_REQUIRE(nbox < (size_t)UINT32_MAX, "cdh_fgr_create: box too large for 32-bit indices");
+      // The box is private to this object, so it is laid out x fastest, as
+      // the callers' gid order runs: a wavefront's 64 consecutive cells then
+      // gather from consecutive box points (the reference's Array3d is z
+      // fastest; only the offsets differ, not the weights or the sums).
+      auto box_index_x = [&](int64_t gid) -> int64_t {
+         const int64_t x = gid % p->nx, y = (gid / p->nx) % p->ny;
+         const int64_t z = gid / ((int64_t)p->nx * p->ny);
+         const int64_t lx = x - g.x0, ly = y - g.y0, lz = z - g.z0;
+         if (lx < 0 || ly < 0 || lz < 0 || lx >= g.bx || ly >= g.by || lz >= g.bz) return -1;
+         return lx + (int64_t)g.bx * (ly + (int64_t)g.by * lz);
+      };
+      int64_t off[19];
+      fgr_offsets_strided(1, g.bx, (int64_t)g.bx * g.by, off);
       std::vector<int64_t> bidx(ntot);
       std::vector<uint8_t> tissue(nbox, 0);
       std::vector<double> sig(6 * nbox, 0.0);
       for (int64_t c = 0; c < ntot; c++) {
-         int64_t ib = box_index(g, p->nx, p->ny, p->gid[c]);
+         int64_t ib = box_index_x(p->gid[c]);
          CDH_REQUIRE(ib >= 0, "cdh_fgr_create: a remote cell lies outside the padded box");
          bidx[c] = ib;
          tissue[ib] = p->cellType[c] > 0 ? 1 : 0;
@@ -308,13 +322,12 @@ int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
          dT.upload(tissue.data(), nbox);
          dS.upload(sig.data(), 6 * nbox);
          dB.upload(bidx.data(), p->nLocal);
-         const int rc = cdh_fgr_weights(g.bx, g.by, g.bz, g.dx, g.dy, g.dz, dT.p, dS.p,
-                                        p->nLocal, dB.p, d->A0.p, d->Aw.p, d->stride, nullptr);
-         if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+         // weights in 64-cell tiles (field k of cell c at blk_index(k, c, 18)):
+         // a wavefront's 18 weight rows are one contiguous 4.5 KB run
+         device_fgr_weights(nullptr, dT.p, dS.p, p->nLocal, dB.p, fgr::weight_geom(g, off),
+                            d->A0.p, d->Aw.p, d->stride, true);
          CDH_HIP(hipDeviceSynchronize());
       }
-      int64_t off[19];
-      fgr_offsets(g.bx, g.by, g.bz, off);
       StencilArgs &a = d->args;
       a.n = (int)p->nLocal;
       a.stride = d->stride;
@@ -325,6 +338,7 @@ int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
       for (int k = 0; k < 19; k++) a.off[k] = (int)off[k];
       a.scale = d->scale;
       a.dVm = nullptr;
+      a.groups_per_xcd = (int)((grid_for(p->nLocal, kBlock) + 7) / 8);
       *out = d.release();
    });
 }
@@ -365,7 +379,7 @@ int cdh_fgr_calc(cdh_fgr *d, double *dVm, void *stream)
       if (d->nLocal == 0) return;
       StencilArgs a = d->args;
       a.dVm = dVm;
-      hipLaunchKernelGGL(k_fgr_calc_f64<false>, dim3(grid_for(d->nLocal, kBlock)), dim3(kBlock), 0,
+      hipLaunchKernelGGL(k_fgr_calc_f64<true>, dim3(8 * a.groups_per_xcd), dim3(kBlock), 0,
                          S(stream), a);
       CDH_LAUNCH_CHECK();
    });

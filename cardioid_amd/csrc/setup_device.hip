@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_fibre_sigma_int(int64_t n, const int
 __global__ __launch_bounds__(kBlock) void k_fgr_weights(const uint8_t *tissue, const double *sigma,
                                                         int64_t n, const int64_t *bidx,
                                                         fgr::WeightGeom G, double *A0, float *Aw,
-                                                        int64_t stride)
+                                                        int64_t stride, int blocked)
 {
    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
    if (c >= n) return;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_fgr_weights(const uint8_t *tissue, c
    fgr::cell_weights(tissue, sigma, bidx[c], G, w);
    A0[c] = fgr::centre_weight(w);
 #pragma unroll
-   for (int k = 1; k < 19; k++) Aw[(k - 1) * stride + c] = w[k];
+   for (int k = 1; k < 19; k++) Aw[blocked ? blk_index(k - 1, c, 18) : (k - 1) * stride + c] = w[k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter_box(int64_t n, const uint32_t *bi, uint32_t b0,
@@ -220,6 +220,16 @@ void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, 
    CDH_HIP(hipStreamSynchronize(stream));
 }
 
+void device_fgr_weights(hipStream_t stream, const uint8_t *tissue, const double *sigma,
+                        int64_t ncell, const int64_t *bidx, const fgr::WeightGeom &G, double *A0,
+                        float *Aw, int64_t stride, bool blocked)
+{
+   if (ncell == 0) return;
+   hipLaunchKernelGGL(k_fgr_weights, dim3(grid_for(ncell)), dim3(kBlock), 0, stream, tissue,
+                      sigma, ncell, bidx, G, A0, Aw, stride, (int)blocked);
+   CDH_LAUNCH_CHECK();
+}
+
 }  // namespace cdh
 
 using namespace cdh;
@@ -282,10 +292,8 @@ int cdh_fgr_weights(int bx, int by, int bz, double dx, double dy, double dz,
                   "cdh_fgr_weights: null argument");
       if (ncell == 0) return;
       BoxGeom g{bx, by, bz, 0, 0, 0, dx, dy, dz};
-      hipLaunchKernelGGL(k_fgr_weights, dim3(grid_for(ncell)), dim3(kBlock), 0,
-                         (hipStream_t)stream, tissue, sigma, ncell, blockIndex,
+      device_fgr_weights((hipStream_t)stream, tissue, sigma, ncell, blockIndex,
                          fgr::weight_geom(g), A0, Aw, stride);
-      CDH_LAUNCH_CHECK();
    });
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "fgr_stencil.hpp"
 #include "fgr_weights.hpp"
 
 namespace cdh {
@@ -21,5 +22,13 @@ void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, 
                          int64_t ntile, const double *sig_cells, const uint32_t *bi, uint32_t b0,
                          int64_t nremote, const uint32_t *remote_bi, const double *remote_sig,
                          float *Aw, float *Aref, uint32_t *Aw16, float *wdelta_scale);
+
+// precomputeCoefficients on the device for ncell cells at box points
+// bidx[] of a box laid out as G.off describes (device arrays; asynchronous
+// on stream): A0[c] and Aw[(k-1)*stride + c], or with `blocked` the 64-cell
+// tile layout Aw[blk_index(k-1, c, 18)]
+void device_fgr_weights(hipStream_t stream, const uint8_t *tissue, const double *sigma,
+                        int64_t ncell, const int64_t *bidx, const fgr::WeightGeom &G, double *A0,
+                        float *Aw, int64_t stride, bool blocked = false);
 
 }  // namespace cdh

@@ -5,9 +5,11 @@ FETCH_SIZE / WRITE_SIZE.
 
     python -m cardioid_amd.profile_step MODE STEPS WAVES BLOCK OCC [shell|batch]
 
-MODE is fp64_fused or mixed_fused; WAVES/BLOCK/OCC the launch configuration
-(register target, workgroup size, occupancy cap).  Meant to run under rocprofv3
-(kernel trace, or one --pmc counter per run).
+MODE is fp64_fused, fp64_fma or mixed_fused (the fused engine; WAVES/BLOCK/OCC
+the launch configuration: register target, workgroup size, occupancy cap), or
+fp64_unfused (the reference loop over the drop-in objects: reaction, scatter,
+FGR stencil, integrator and sensor kernels; WAVES/BLOCK/OCC ignored).  Meant
+to run under rocprofv3 (kernel trace, or one --pmc counter per run).
 """
 from __future__ import annotations
 
@@ -30,6 +32,15 @@ def main(argv):
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
+    if mode == "fp64_unfused":
+        anat, depth = W.ellipsoid_shell(256, 256, 512, seed=0)
+        stims = W.endocardial_point_stimuli(anat, depth, 8, seed=0)
+        sim = hip.Simulation(anat, W.CELLTYPES_EMI, 0.02, W.DIFFUSION_SCALE, stims, mode=mode)
+        sim.run(steps)
+        sim.sync()
+        sim.close()
+        print(f"profile_step: {mode} shell, {steps} steps on {anat.ncell} cells", flush=True)
+        return
     if work == "shell":
         anat, depth = W.ellipsoid_shell(256, 256, 512, seed=0)
         stims = W.endocardial_point_stimuli(anat, depth, 8, seed=0)
