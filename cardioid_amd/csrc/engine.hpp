@@ -29,9 +29,6 @@ namespace cdh {
 
 constexpr int kSlabStim = 8;                 // box stimuli per slab (mask bits)
 constexpr uint16_t kNoSlab = 0xFFFF;         // tile not in use
-// stencil byte offsets are stored shifted by this many box points so that
-// they are non-negative (the buffer instructions' SGPR offset is unsigned)
-constexpr int64_t kBoxShift = int64_t(1) << 22;
 
 // per-cell metadata word
 constexpr uint32_t kMetaPad = 0x8000u;       // padding cell of a slab's last tile
@@ -202,18 +199,6 @@ __device__ __forceinline__ void engine_range(const EngineArgs &a, const SlabDev 
 
 // Buffer resource over a box so that (box index)*8 is the VGPR offset and
 // a shifted stencil offset is the SGPR offset.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(const double *box)
-{
-   // raw buffer, no stride, the whole 32-bit range (the host keeps arena
-   // boxes below 2^28 points); third dword as for every gfx9 target
-   return __builtin_amdgcn_make_buffer_rsrc((void *)(box - kBoxShift), (short)0, -1, 0x00020000);
-}
-
-__device__ __forceinline__ double box_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
-{
-   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-
 // kernels (engine_f64.hip, engine_mixed.hip); WAVES = target waves per SIMD
 template <int WAVES, bool FMA>
 __global__ void k_engine_f64(EngineArgs a);

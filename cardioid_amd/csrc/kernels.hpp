@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "box_access.hpp"
 #include "tt06_model.hpp"
 
 namespace cdh {
@@ -21,7 +22,8 @@ struct ReactionArgs {
    const ParamSet *params;      // parameter sets
    const uint8_t *pid;          // per-cell parameter set, or null (set 0)
    const uint8_t *flags;        // per-cell gate flags
-   double *state;               // [19][stride]
+   double *state;               // [19][stride], or 64-cell tiles (blk_index) for the BLK
+                                // kernel and k_tt06_gate_f64
    int64_t stride;
    int n;
    const int *index;            // reaction-local -> array index, or null (identity)

@@ -45,7 +45,8 @@ __global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
 template __global__ void k_tt06_calc_f64<false>(ReactionArgs);
 template __global__ void k_tt06_calc_f64<true>(ReactionArgs);
 
-// Gate update only (Reaction::updateGate of the pdr loop): the 12 gates at Vm.
+// Gate update only (Reaction::updateGate of the pdr loop): the 12 gates at
+// Vm; states in 64-cell tiles.
 __global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -54,11 +55,11 @@ __global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
    const double V = a.Vm[e];
    double st[N_STATE];
 #pragma unroll
-   for (int s = S_M; s < N_STATE; s++) st[s] = a.state[s * a.stride + i];
+   for (int s = S_M; s < N_STATE; s++) st[s] = a.state[blk_index(s, i, N_STATE)];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
    gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
-   for (int s = S_M; s < N_STATE; s++) a.state[s * a.stride + i] = st[s];
+   for (int s = S_M; s < N_STATE; s++) a.state[blk_index(s, i, N_STATE)] = st[s];
 }
 
 __global__ __launch_bounds__(256) void k_scatter_f64(int n, const uint32_t *__restrict__ bi,
@@ -78,7 +79,6 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
    const int64_t i =
        ((int64_t)xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * blockDim.x + threadIdx.x;
    if (i >= a.n) return;
-   const double *phi = a.box + a.bi[i];
    // the weights stream once per step; the centre weight is minus their sum
    // in order, in double (FGRDiffusionOMP.cc:214-219), recomputed bit for bit
    float w[18];
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
       w[k] = __builtin_nontemporal_load(a.Aw + sidx<BLK>(k, i, 18, a.stride));
       A0 -= w[k];
    }
+   const double *phi = a.box + a.bi[i];
    double tmp = A0 * phi[0];
 #pragma unroll
    for (int k = 1; k < 19; k++) tmp += (double)w[k - 1] * phi[a.off[k]];

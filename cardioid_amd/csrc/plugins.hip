@@ -138,9 +138,14 @@ int cdh_tt06_create(const cdh_tt06_params *p, int numPoints, cdh_tt06 **out)
       r->flags.upload(fl.data(), fl.size());
       r->flagsAllFixed.alloc(r->stride);
       r->flagsAllFixed.upload(fixed.data(), fixed.size());
-      std::vector<double> st = rest_state_soa(r->ct, numPoints, r->stride, r->K.c9);
-      r->state.alloc(st.size());
-      r->state.upload(st.data(), st.size());
+      // states in 64-cell tiles (state s of cell i at blk_index(s, i, 19)): a
+      // wavefront's 19 rows are one contiguous 9.5 KB run
+      const std::vector<double> st = rest_state_soa(r->ct, numPoints, r->stride, r->K.c9);
+      std::vector<double> tiled(st.size());
+      for (int s = 0; s < N_STATE; s++)
+         for (int64_t i = 0; i < r->stride; i++) tiled[blk_index(s, i, N_STATE)] = st[s * r->stride + i];
+      r->state.alloc(tiled.size());
+      r->state.upload(tiled.data(), tiled.size());
       *out = r.release();
    });
 }
@@ -160,7 +165,7 @@ int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const do
       CDH_REQUIRE(n == r->n, "cdh_tt06_calc: n differs from numPoints");
       if (n == 0) return;
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVm, r->flags.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(k_tt06_calc_f64<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -173,7 +178,7 @@ int cdh_tt06_update_nongate(cdh_tt06 *r, double dt, const int *indexArray, int n
       if (n == 0) return;
       // the calc kernel with every gate held fixed is the non-gate update
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVR, r->flagsAllFixed.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(k_tt06_calc_f64<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -232,7 +237,8 @@ int cdh_tt06_get_value(cdh_tt06 *r, int iCell, int handle, double *value)
    return guard([&] {
       CDH_REQUIRE(r && value && iCell >= 0 && iCell < r->n && handle >= 0 && handle < N_STATE,
                   "cdh_tt06_get_value: bad arguments");
-      r->state.download(value, 1, handle * r->stride + iCell);
+      CDH_HIP(hipDeviceSynchronize());
+      r->state.download(value, 1, blk_index(handle, iCell, N_STATE));
    });
 }
 
@@ -241,7 +247,8 @@ int cdh_tt06_set_value(cdh_tt06 *r, int iCell, int handle, double value)
    return guard([&] {
       CDH_REQUIRE(r && iCell >= 0 && iCell < r->n && handle >= 0 && handle < N_STATE,
                   "cdh_tt06_set_value: bad arguments");
-      r->state.upload(&value, 1, handle * r->stride + iCell);
+      CDH_HIP(hipDeviceSynchronize());
+      r->state.upload(&value, 1, blk_index(handle, iCell, N_STATE));
    });
 }
 
@@ -250,7 +257,10 @@ int cdh_tt06_get_state(cdh_tt06 *r, double *host)
    return guard([&] {
       CDH_REQUIRE(r && host, "cdh_tt06_get_state: null argument");
       CDH_HIP(hipDeviceSynchronize());
-      for (int s = 0; s < N_STATE; s++) r->state.download(host + (size_t)s * r->n, r->n, s * r->stride);
+      std::vector<double> tiled((size_t)N_STATE * r->stride);
+      r->state.download(tiled.data(), tiled.size());
+      for (int s = 0; s < N_STATE; s++)
+         for (int64_t i = 0; i < r->n; i++) host[(size_t)s * r->n + i] = tiled[blk_index(s, i, N_STATE)];
    });
 }
 
@@ -259,7 +269,11 @@ int cdh_tt06_set_state(cdh_tt06 *r, const double *host)
    return guard([&] {
       CDH_REQUIRE(r && host, "cdh_tt06_set_state: null argument");
       CDH_HIP(hipDeviceSynchronize());
-      for (int s = 0; s < N_STATE; s++) r->state.upload(host + (size_t)s * r->n, r->n, s * r->stride);
+      std::vector<double> tiled((size_t)N_STATE * r->stride);
+      r->state.download(tiled.data(), tiled.size());
+      for (int s = 0; s < N_STATE; s++)
+         for (int64_t i = 0; i < r->n; i++) tiled[blk_index(s, i, N_STATE)] = host[(size_t)s * r->n + i];
+      r->state.upload(tiled.data(), tiled.size());
    });
 }
 
