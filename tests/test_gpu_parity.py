@@ -185,6 +185,15 @@ def test_engine_brick_matches_oracle():
     _engine_vs_oracle(a, ["endoCellML"], [W.box_stimulus(xMax=4, nx=16, ny=12, nz=10)], 1000)
 
 
+def test_cfg0_full_size_matches_oracle():
+    """The CPU-runnable configuration itself: 64^3 all-tissue endocardial
+    brick, fibres along x (sigma_l 0.1334, sigma_t 0.0176 mS/mm), box stimulus
+    x < 4 of -52 for 1 ms, 1000 steps of 0.02 ms (262,144 cells)."""
+    say("engine cfg0: brick 64^3 endo, box stimulus x < 4, 1000 steps")
+    a = W.brick(64, 64, 64)
+    _engine_vs_oracle(a, ["endoCellML"], [W.box_stimulus(xMax=4, nx=64, ny=64, nz=64)], 1000)
+
+
 def test_engine_shell_three_celltypes_matches_oracle():
     say("engine ellipsoidal shell 32x28x40, endo/mid/epi, rotating fibres, 8 stimuli")
     a, depth = W.ellipsoid_shell(32, 28, 40, seed=0)
