@@ -39,6 +39,7 @@ UNFUSED_F64_BYTES = 480
 # 18 FP16 differences (36) and the tile's 18 float references shared by 64 cells (1.125)
 MIXED_BYTES = 221.125
 DT = 0.02
+FLOP_PER_CELL_STEP = 1060
 
 
 def log(msg):
@@ -81,12 +82,34 @@ def cpu_baseline(budget_s: float = 12.0):
                       f"{el:.1f} s, oracle/liboracle.so with {threads} OpenMP threads on {cpu}"}
 
 
-def load_traffic(stem):
+def load_profile(stem):
     path = os.path.join(ROOT, "profiles", stem + "_traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        return json.load(fh)
+
+
+def load_traffic(stem):
+    return load_profile(stem).get("bytes_per_launch")
+
+
+def valu(stem, ncell, ms_per_step, peak_tflops=78.6, simds=1024, clock_ghz=2.4):
+    """Arithmetic beside the HBM roofline: SURVEY's 1,060 flop per cell-step
+    against the FP64 vector peak, and the VALU issue fraction from rocprof's
+    VALU instructions per wave (4 SIMD cycles each) at this run's step time."""
+    p = load_profile(stem)
+    rate = ncell / (ms_per_step * 1e-3)
+    out = {"flop_per_cell_step": FLOP_PER_CELL_STEP,
+           "achieved_tflops": FLOP_PER_CELL_STEP * rate / 1e12, "peak_tflops": peak_tflops,
+           "flop_frac": FLOP_PER_CELL_STEP * rate / (peak_tflops * 1e12)}
+    ipw = p.get("valu_instructions_per_wave")
+    if ipw:
+        waves = (ncell + 63) // 64
+        out["valu_instructions_per_wave"] = ipw
+        out["issue_frac_at_peak_clock"] = (4 * ipw * waves /
+                                           (simds * clock_ghz * 1e9 * ms_per_step * 1e-3))
+    return out
 
 
 def fused_run(hip, slab, ncell, mode, steps, warmup, tune_steps, graph):
@@ -280,6 +303,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic("engine_f64_fma_shell256x256x512"),
                      "kernel": "k_engine_f64", "algorithmic_bytes_per_cell_step": FUSED_F64_BYTES},
+        "valu": valu("engine_f64_fma_shell256x256x512", anat.ncell, ms_per_step),
     }
     mms = mx["ms"] / mx["steps"]
     both = (f64["act"] >= 0) & (mx["act"] >= 0)
@@ -290,7 +314,8 @@ def main():
                    "occupancy": mx["launch"][2]},
         "algorithmic_bytes_per_cell_step": MIXED_BYTES,
         "achieved_GBps": MIXED_BYTES * anat.ncell / (mms * 1e-3) / 1e9,
-        "traffic": load_traffic("engine_mixed_shell256x256x512")}
+        "traffic": load_traffic("engine_mixed_shell256x256x512"),
+        "valu": valu("engine_mixed_shell256x256x512", anat.ncell, mms)}
     if f64["steps"] == mx["steps"]:
         out["mixed_fused"]["error_vs_fp64_at_end"] = {
             "t_ms": (args.warmup + f64["steps"]) * DT,

@@ -69,6 +69,14 @@ def main(argv=None):
     ap.add_argument("--cmd-write", default="")
     ap.add_argument("--note", default="")
     ap.add_argument("--copies", type=int, default=3)
+    ap.add_argument("--valu", help="rocprofv3 --pmc run of SQ_WAVES SQ_INSTS_VALU "
+                    "SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES GRBM_GUI_ACTIVE GRBM_COUNT")
+    ap.add_argument("--cmd-valu", default="")
+    ap.add_argument("--flops", type=float, default=1060.0,
+                    help="algorithmic floating-point operations per cell-step (FMA = 2)")
+    ap.add_argument("--peak-tflops", type=float, default=78.6)
+    ap.add_argument("--simds", type=int, default=1024)
+    ap.add_argument("--clock-ghz", type=float, default=2.4)
     a = ap.parse_args(argv)
 
     rows = trace_rows(a.trace)
@@ -113,6 +121,29 @@ def main(argv=None):
                   f"(algorithmic {a.bytes:g}) |", ""]
         out.update(fetch_bytes_per_launch=fb, write_bytes_per_launch=wb,
                    bytes_per_launch=fb + wb, fetch_scale=fs, write_scale=ws)
+    if a.valu:
+        cnt = {c: counter_means(a.valu, c) for c in ("SQ_WAVES", "SQ_INSTS_VALU",
+                                                     "SQ_ACTIVE_INST_VALU")}
+        waves = cnt["SQ_WAVES"][_pick(cnt["SQ_WAVES"], a.kernel)][0]
+        insts = cnt["SQ_INSTS_VALU"][_pick(cnt["SQ_INSTS_VALU"], a.kernel)][0]
+        active = cnt["SQ_ACTIVE_INST_VALU"][_pick(cnt["SQ_ACTIVE_INST_VALU"], a.kernel)][0]
+        # a wave64 VALU instruction occupies its SIMD for 4 cycles
+        avail = a.simds * a.clock_ghz * 1e9 * avg_us * 1e-6
+        issue = 4 * insts / avail
+        flop_frac = a.flops * a.cells / (avg_us * 1e-6) / (a.peak_tflops * 1e12)
+        lines += ["## VALU", "", "Command:", "", f"    {a.cmd_valu}", "",
+                  "| | per launch | per wave |", "|---|---|---|",
+                  f"| SQ_WAVES | {waves:.4g} | |",
+                  f"| SQ_INSTS_VALU | {insts:.4g} | {insts / waves:.0f} |",
+                  f"| SQ_ACTIVE_INST_VALU | {active:.4g} | {active / waves:.0f} |", "",
+                  f"VALU issue: 4 cycles x {insts:.4g} instructions over {a.simds} SIMDs x "
+                  f"{a.clock_ghz} GHz x {avg_us:.1f} us = **{issue:.2f}** of the SIMDs' cycles "
+                  f"(at the peak clock; a lower clock raises it).  Algorithmic arithmetic: "
+                  f"{a.flops:g} flop per cell-step = {flop_frac * a.peak_tflops:.1f} TFLOP/s = "
+                  f"{flop_frac:.3f} of {a.peak_tflops} TFLOP/s.", ""]
+        out.update(valu_instructions_per_wave=insts / waves, valu_issue_frac=issue,
+                   flop_per_cell_step=a.flops, flop_frac=flop_frac,
+                   peak_tflops=a.peak_tflops, clock_ghz=a.clock_ghz)
     with open(a.out + ".md", "w") as fh:
         fh.write("\n".join(lines))
     with open(a.out + "_traffic.json", "w") as fh:
