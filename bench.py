@@ -202,17 +202,18 @@ def shards(hip, W, anat, stims, parts, mode, launch, steps, graph):
             "largest_share_halo_bytes_per_step": 8 * slabs[big].nremote}
 
 
-def serving(hip, W, nreq, rate, hbm_gb, graph):
+def serving(hip, W, nreq, rate, hbm_gb, graph, max_active_cells=None):
     from cardioid_amd import serve
     reqs = W.request_stream(nreq, rate_hz=rate, seed=4)
     cells, box = serve.arena_for_budget(int(hbm_gb * 2 ** 30))
     b = hip.Batch(DT, cells, box, slab_capacity=1024, mode="mixed_fused", sensors=3)
     b.set_launch(3, 64)
     b.set_graph(graph)
-    st = serve.Server(b, chunk=100).serve(reqs)
+    st = serve.Server(b, chunk=100, max_active_cells=max_active_cells).serve(reqs)
     b.close()
     s = st.summary()
     s.update({"rate_hz": rate, "hbm_budget_gb": hbm_gb, "arena_cells": cells,
+              "max_active_cells": max_active_cells,
               "request_dims": "each axis log-uniform 32..128, duration 200-1000 ms, "
                               "scales U[0,1], mixed precision"})
     return s
@@ -233,6 +234,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
+    ap.add_argument("--serve-cap", type=int, default=6000000)
     ap.add_argument("--shard-parts", type=int, default=8)
     ap.add_argument("--shard-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -358,6 +360,14 @@ def main():
                                  args.graph)
         log(f"serving: p50 {out['serving']['p50_latency_s']:.2f} s, "
             f"p99 {out['serving']['p99_latency_s']:.2f} s")
+        if args.serve_cap > 0:
+            # the same stream with at most serve_cap cells in flight: requests
+            # finish in arrival order (lower median, same tail)
+            out["serving_capped"] = serving(hip, W, args.serve_requests, args.serve_rate,
+                                            args.serve_hbm_gb, args.graph, args.serve_cap)
+            log(f"serving, {args.serve_cap} cells in flight at most: p50 "
+                f"{out['serving_capped']['p50_latency_s']:.2f} s, p99 "
+                f"{out['serving_capped']['p99_latency_s']:.2f} s")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores)")
         out["cpu_baseline"] = cpu_baseline()

@@ -93,11 +93,18 @@ class Server:
     the same methods)."""
 
     def __init__(self, batch, dt: float = 0.02, chunk: int = 100,
-                 make_slab: Callable = request_slab):
+                 make_slab: Callable = request_slab, max_active_cells: Optional[int] = None):
+        """max_active_cells: admit a request only while the cells in flight
+        stay within this many (one request is always admitted when none is in
+        flight).  Enough cells to fill the GPU keep its throughput, and
+        requests then finish in arrival order instead of all sharing the GPU
+        to the end, which lowers the median latency of an overloaded stream.
+        None: the arena alone limits admission."""
         self.batch = batch
         self.dt = dt
         self.chunk = chunk
         self.make_slab = make_slab
+        self.max_active_cells = max_active_cells
 
     def serve(self, requests: List[W.Request], clock=time.perf_counter,
               sleep=time.sleep) -> ServeStats:
@@ -125,6 +132,9 @@ class Server:
                 if t.prepared is None:
                     t.prepared = self.make_slab(t.request)
                 a, slab = t.prepared
+                if (self.max_active_cells and active and
+                        sum(x.ncell for x in active.values()) + a.ncell > self.max_active_cells):
+                    return
                 try:
                     t.slab = self.batch.add(slab)
                 except hip.ArenaFull:
@@ -194,15 +204,17 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--graph", type=int, default=10)
+    ap.add_argument("--max-active-cells", type=int, default=0,
+                    help="cells in flight at most (0: the HBM budget alone)")
     args = ap.parse_args(argv)
     reqs = W.request_stream(args.requests, rate_hz=args.rate, seed=args.seed)
     cells, box = arena_for_budget(int(args.hbm_gb * 2 ** 30))
     b = hip.Batch(0.02, cells, box, slab_capacity=1024, mode="mixed_fused", sensors=3)
     b.set_graph(args.graph)
-    stats = Server(b, chunk=args.chunk).serve(reqs)
+    stats = Server(b, chunk=args.chunk, max_active_cells=args.max_active_cells or None).serve(reqs)
     out = stats.summary()
     out.update(hbm_budget_gb=args.hbm_gb, arena_cells=cells, arena_box=box,
-               rate_hz=args.rate, info=b.info())
+               rate_hz=args.rate, max_active_cells=args.max_active_cells, info=b.info())
     b.close()
     print(json.dumps(out))
 
