@@ -39,7 +39,6 @@ UNFUSED_F64_BYTES = 480
 # 18 FP16 differences (36) and the tile's 18 float references shared by 64 cells (1.125)
 MIXED_BYTES = 221.125
 DT = 0.02
-FLOP_PER_CELL_STEP = 1060
 
 
 def log(msg):
@@ -98,6 +97,8 @@ def valu(stem, ncell, ms_per_step, peak_tflops=78.6, simds=1024, clock_ghz=2.4):
     """Arithmetic beside the HBM roofline: SURVEY's 1,060 flop per cell-step
     against the FP64 vector peak, and the VALU issue fraction from rocprof's
     VALU instructions per wave (4 SIMD cycles each) at this run's step time."""
+    from cardioid_amd.flops import flop_per_cell_step
+    FLOP_PER_CELL_STEP = flop_per_cell_step()
     p = load_profile(stem)
     rate = ncell / (ms_per_step * 1e-3)
     out = {"flop_per_cell_step": FLOP_PER_CELL_STEP,

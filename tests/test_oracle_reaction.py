@@ -225,3 +225,15 @@ def test_gate_flags_rule():
     # v2 loops stop at 4*floor(27/4) = 24; the f-gate main loop covers 0..23
     assert [x & 1 for x in f] == [0] * 24 + [1] * 3
     assert [(x >> 1) & 1 for x in f] == ([1] * 4 + [0] * 8) * 2 + [0] * 3
+
+
+def test_flop_count_rederived_from_the_fit_orders():
+    """1,060 flop per endocardial cell-step: 66 Horner multiply-adds in the
+    non-gate fits and 303 in the 24 gate fits of the fixture, plus the fixed
+    arithmetic; the s1 fits of mid/epi cells add 9 multiply-adds."""
+    from cardioid_amd import flops
+    b0, b1 = flops.breakdown(s_switch=0), flops.breakdown(s_switch=1)
+    assert (b0["nongate_fit_fmas"], b0["gate_fit_fmas"]) == (66, 303)
+    assert b0["nongate"] == 309 and b0["gates"] == 706 and b0["reaction"] == 1019
+    assert b0["total"] == flops.flop_per_cell_step() == 1060
+    assert b1["gate_fit_fmas"] - b0["gate_fit_fmas"] == 9
