@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--unfused-steps", type=int, default=500)
     ap.add_argument("--mixed-steps", type=int, default=5000)
     ap.add_argument("--exact-steps", type=int, default=2000)
-    ap.add_argument("--tune-steps", type=int, default=20)
+    ap.add_argument("--tune-steps", type=int, default=40)
     ap.add_argument("--graph", type=int, default=10)
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=40)
