@@ -91,6 +91,8 @@ void slab_extent(const SlabInput &in, int64_t *ntile, int64_t *nbox)
    const BoxGeom g = bounding_box(in.nx, in.ny, in.nz, in.gid, in.ncell, in.dx, in.dy, in.dz);
    *ntile = (in.ncell + kBlk - 1) / kBlk;
    *nbox = (int64_t)g.bx * g.by * g.bz;
+   CDH_REQUIRE(*nbox < (int64_t(1) << 28), "slab: box above 2^28 points");
+   CDH_REQUIRE((int64_t)g.by * g.bz + g.bz + 1 < kBoxShift, "slab: box plane above 2^22 points");
 }
 
 SlabHost prepare_slab(const SlabInput &in, double c9)

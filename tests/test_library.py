@@ -90,3 +90,45 @@ def test_null_arguments_are_errors_not_crashes():
     assert L.cdh_fgr_calc(None, None, None) == 1
     assert L.cdh_sim_run(None, 1) == 1
     assert L.cdh_fill(-1, 0.0, None, None) == 1
+
+
+def _slab(gid, n=(4, 4, 4), **kw):
+    from cardioid_amd import workloads as W
+    gid = np.asarray(gid, dtype=np.int64)
+    a = W.Anatomy(n[0], n[1], n[2], 0.2, 0.2, 0.2, gid, np.zeros(gid.size, np.uint8),
+                  np.tile([1e-4, 0, 0, 1e-4, 0, 1e-4], (gid.size, 1)))
+    return hip.Slab(a, ["endoCellML"], 714.0, **kw)
+
+
+def test_slab_limits_are_host_errors():
+    """Slab descriptions the engine cannot hold fail on the host, before any
+    device work: a padded box of 2^28 points or more, a box plane of 2^22
+    points or more, cells outside the grid, bad cell-type indices, more than 8
+    stimuli, remote cells outside the grid."""
+    from cardioid_amd import workloads as W
+    ok = _slab([0, 1, 2])
+    assert ok.extent()[:2] == (64, 5 * 3 * 3)
+    n = (700, 700, 700)
+    with pytest.raises(hip.CardioidHipError, match="2\\^28"):
+        _slab([0, 700 ** 3 - 1], n=n).extent()
+    n = (1, 2098, 2098)       # box 3 x 2100 x 2100: a y-z plane of 4.41 M points
+    with pytest.raises(hip.CardioidHipError, match="2\\^22"):
+        _slab([0, 2098 * 2098 - 1], n=n).extent()
+    with pytest.raises(hip.CardioidHipError, match="outside the grid"):
+        _slab([0, 64]).extent()
+    bad = _slab([0, 1])
+    bad._keep[2][1] = 3       # cell-type index beyond the one type given
+    with pytest.raises(hip.CardioidHipError, match="cell type index"):
+        bad.extent()
+    stims = [W.box_stimulus(nx=4, ny=4, nz=4)] * 9
+    with pytest.raises(hip.CardioidHipError, match="8 stimuli"):
+        _slab([0, 1], stimuli=stims).extent()
+    with pytest.raises(hip.CardioidHipError, match="remote gid"):
+        _slab([0, 1], remote_gid=[64], remote_sigma=np.zeros((1, 6))).extent()
+
+
+def test_checkpoint_of_no_cells(tmp_path):
+    path = hip.write_checkpoint(str(tmp_path), 3, 0.06, (4, 4, 4), np.zeros(0, np.int64),
+                                np.zeros(0), np.zeros((19, 0)))
+    vm, st, loop, time = hip.read_checkpoint(path, np.zeros(0, np.int64))
+    assert vm.size == 0 and st.shape == (19, 0) and (loop, time) == (3, 0.06)
