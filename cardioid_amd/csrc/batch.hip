@@ -1,5 +1,7 @@
 // batch.hip -- C ABI of the multi-slab engine (cdh_batch_*).
+#include <array>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -13,6 +15,9 @@ using namespace cdh;
 struct cdh_batch {
    std::unique_ptr<Engine> eng;
    TT06Const K{};
+   // per slab: the caller's gids (input order) and grid, for checkpoints
+   std::map<int, std::vector<int64_t>> gid;
+   std::map<int, std::array<int, 3>> grid;
 };
 
 namespace {
@@ -131,6 +136,8 @@ int cdh_batch_add(cdh_batch *b, const cdh_slab_desc *d, int *slab)
       const SlabHost h = prepare_slab(in, b->K.c9);
       const int id = e.add(h);
       if (id < 0) throw Error(CDH_ERR_FULL, "cdh_batch_add: arena full");
+      b->gid[id].assign(d->gid, d->gid + d->ncell);
+      b->grid[id] = {d->nx, d->ny, d->nz};
       *slab = id;
    });
    return rc;
@@ -138,7 +145,59 @@ int cdh_batch_add(cdh_batch *b, const cdh_slab_desc *d, int *slab)
 
 int cdh_batch_remove(cdh_batch *b, int slab)
 {
-   return guard([&] { engine(b).remove(slab); });
+   return guard([&] {
+      engine(b).remove(slab);
+      b->gid.erase(slab);
+      b->grid.erase(slab);
+   });
+}
+
+int cdh_batch_write_checkpoint(cdh_batch *b, int slab, const char *dir, int ascii,
+                               char *state_path, int path_len)
+{
+   return guard([&] {
+      CDH_REQUIRE(dir, "cdh_batch_write_checkpoint: null directory");
+      Engine &e = engine(b);
+      const Engine::Slab &s = e.slab(slab);
+      const int64_t n = s.n;
+      std::vector<double> vm(n), st((size_t)N_STATE * n);
+      const std::vector<double> v = e.vm(slab), q = e.state19(slab);
+      scatter(s, v.data(), vm.data());
+      scatter(s, q.data(), st.data(), N_STATE);
+      const auto &g = b->grid.at(slab);
+      const int rc = cdh_checkpoint_write(dir, "simulate", s.loop, s.time, g[0], g[1], g[2], n,
+                                          b->gid.at(slab).data(), vm.data(), st.data(), ascii,
+                                          state_path, path_len);
+      if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+   });
+}
+
+int cdh_batch_read_checkpoint(cdh_batch *b, int slab, const char *state_file)
+{
+   return guard([&] {
+      CDH_REQUIRE(state_file, "cdh_batch_read_checkpoint: null file name");
+      Engine &e = engine(b);
+      const Engine::Slab &s = e.slab(slab);
+      const int64_t n = s.n;
+      // input order; fields the file lacks keep their values
+      std::vector<double> vm(n), st((size_t)N_STATE * n);
+      const std::vector<double> v0 = e.vm(slab), q0 = e.state19(slab);
+      scatter(s, v0.data(), vm.data());
+      scatter(s, q0.data(), st.data(), N_STATE);
+      int loop = 0;
+      double time = 0.0;
+      const int rc = cdh_checkpoint_read(state_file, n, b->gid.at(slab).data(), vm.data(),
+                                         st.data(), &loop, &time);
+      if (rc != CDH_OK) throw Error(rc, cdh_last_error());
+      std::vector<double> vs(n), qs((size_t)N_STATE * n);
+      for (int64_t k = 0; k < n; k++) {
+         vs[k] = vm[s.perm[k]];
+         for (int q = 0; q < N_STATE; q++) qs[q * n + k] = st[q * n + s.perm[k]];
+      }
+      e.set_state19(slab, qs, vs);
+      e.set_vm(slab, vs);
+      e.set_clock(slab, time, loop);
+   });
 }
 
 int cdh_batch_link_halo(cdh_batch *b, int dst, int src, int64_t *npoints)

@@ -109,7 +109,8 @@ EXPORTS = [
     "cdh_batch_range_report", "cdh_batch_slab_info", "cdh_batch_info",
     "cdh_checkpoint_write", "cdh_checkpoint_read", "cdh_sim_write_checkpoint",
     "cdh_sim_read_checkpoint", "cdh_fibre_conductivity", "cdh_fibre_conductivity_int",
-    "cdh_fgr_weights", "cdh_batch_link_halo",
+    "cdh_fgr_weights", "cdh_batch_link_halo", "cdh_batch_write_checkpoint",
+    "cdh_batch_read_checkpoint",
 ]
 
 ERR_FULL = 5
@@ -187,6 +188,9 @@ def lib():
         L.cdh_batch_add.argtypes = [C.c_void_p, C.POINTER(SlabDesc), ip]
         L.cdh_batch_remove.argtypes = [C.c_void_p, C.c_int]
         L.cdh_batch_link_halo.argtypes = [C.c_void_p, C.c_int, C.c_int, i64p]
+        L.cdh_batch_write_checkpoint.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int,
+                                                 C.c_char_p, C.c_int]
+        L.cdh_batch_read_checkpoint.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
         L.cdh_batch_run.argtypes = [C.c_void_p, C.c_int]
         L.cdh_batch_run_timed.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_sync.argtypes = [C.c_void_p]
@@ -500,6 +504,19 @@ class Batch:
     def remove(self, sid: int):
         check(lib().cdh_batch_remove(self.h, sid), "cdh_batch_remove")
         self.ncell.pop(sid, None)
+
+    def write_checkpoint(self, sid: int, directory: str, ascii: bool = True) -> str:
+        """snapshot.<loop>/state#000000 and restart of one slab; returns the
+        stateFile name."""
+        out = C.create_string_buffer(4096)
+        check(lib().cdh_batch_write_checkpoint(self.h, sid, directory.encode(), int(bool(ascii)),
+                                               out, len(out)), "cdh_batch_write_checkpoint")
+        return out.value.decode()
+
+    def read_checkpoint(self, sid: int, state_file: str):
+        """States, Vm and clock of slab `sid` from a stateFile."""
+        check(lib().cdh_batch_read_checkpoint(self.h, sid, state_file.encode()),
+              "cdh_batch_read_checkpoint")
 
     def link_halo(self, dst: int, src: int) -> int:
         """After every step, slab dst's remote cells owned by slab src take

@@ -359,6 +359,12 @@ int cdh_batch_remove(cdh_batch *b, int slab);
  * their current Vm now).  Link every (dst, src) pair of neighbours; a slab's
  * links end when either slab is removed.  *npoints: points linked. */
 int cdh_batch_link_halo(cdh_batch *b, int dst, int src, int64_t *npoints);
+/* checkpoint / restart of one slab, as cdh_sim_write_checkpoint /
+ * cdh_sim_read_checkpoint (the slab's cells in the order they were added,
+ * its own clock): a served request can be written out, removed and resumed */
+int cdh_batch_write_checkpoint(cdh_batch *b, int slab, const char *dir, int ascii,
+                               char *state_path, int path_len);
+int cdh_batch_read_checkpoint(cdh_batch *b, int slab, const char *state_file);
 /* advances every slab by nsteps; asynchronous on the batch's stream */
 int cdh_batch_run(cdh_batch *b, int nsteps);
 int cdh_batch_run_timed(cdh_batch *b, int nsteps, double *ms);

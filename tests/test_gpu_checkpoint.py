@@ -86,3 +86,44 @@ def test_restart_continues_the_run(tmp_path, mode, restart_mode, ascii, tol):
         np.testing.assert_array_equal(st, st_want)
     else:
         assert dv <= tol
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64_fused", 0.0), ("mixed_fused", 1e-6)])
+def test_batch_slab_evicted_and_resumed(tmp_path, mode, tol):
+    """A slab of the batched engine written out at step 150, removed while
+    the other slab runs on, added back into a different arena slot and read
+    back, then run to step 300, ends where an uninterrupted run ends (bit for
+    bit in FP64; the mixed form as in the single-engine test)."""
+    cases = W.batched_slabs(n=2, seed=7, lo=(24, 24, 24), hi=(40, 40, 40))
+    slabs = [hip.Slab(c.anatomy, ["endoCellML"], W.DIFFUSION_SCALE, c.stimuli, scales=c.scales)
+             for c in cases]
+    ext = [s.extent() for s in slabs]
+    cells, boxes = sum(e[0] for e in ext), sum(e[1] for e in ext)
+    say(f"batch slab eviction and resume: {mode}, slabs of {[s.ncell for s in slabs]} cells")
+    b = hip.Batch(DT, cells, boxes, slab_capacity=2, mode=mode)
+    ids = [b.add(s) for s in slabs]
+    b.run(300)
+    want = b.vm(ids[1])
+    b.close()
+
+    b = hip.Batch(DT, cells, boxes, slab_capacity=2, mode=mode)
+    ids = [b.add(s) for s in slabs]
+    b.set_graph(10)
+    b.run(150)
+    path = b.write_checkpoint(ids[1], str(tmp_path), ascii=False)
+    b.remove(ids[1])
+    b.run(50)
+    again = b.add(slabs[1])
+    b.read_checkpoint(again, path)
+    assert b.slab_info(again)["loop"] == 150
+    b.run(150)
+    assert b.slab_info(again)["loop"] == 300
+    got = b.vm(again)
+    b.close()
+    dv = np.abs(got - want).max()
+    say(f"  max |dVm| {dv:.3e} mV")
+    assert want.max() > 0.0
+    if tol == 0.0:
+        assert np.array_equal(got, want)
+    else:
+        assert dv <= tol
