@@ -178,3 +178,24 @@ def test_active_cell_cap_keeps_order_and_lowers_the_median():
     free, capped = res[None], res[2 * 128 ** 3]
     assert capped["p50_latency_s"] < 0.8 * free["p50_latency_s"]
     assert capped["wall_s"] <= 1.05 * free["wall_s"]
+
+
+def test_least_work_first_admission():
+    """order="shortest" admits the waiting request with the least cells x
+    steps first: every request still completes with its exact steps, the
+    cap still holds, and the median latency of an overloaded stream drops
+    below arrival order's."""
+    reqs = W.request_stream(30, rate_hz=20.0, seed=11)
+    res = {}
+    for order in ("arrival", "shortest"):
+        clock = FakeClock()
+        b = FakeBatch(64 * 128 ** 3, clock)
+        st = serve.Server(b, dt=0.02, chunk=50, make_slab=fake_make_slab,
+                          max_active_cells=2 * 128 ** 3, order=order).serve(
+            reqs, clock=clock, sleep=clock.sleep)
+        assert len(st.tickets) == 30
+        assert all((t.tact == t.nsteps).all() for t in st.tickets)
+        res[order] = st.summary()
+    assert res["shortest"]["p50_latency_s"] < res["arrival"]["p50_latency_s"]
+    with pytest.raises(ValueError):
+        serve.Server(None, order="random")
