@@ -22,9 +22,7 @@ struct ReactionArgs {
    const ParamSet *params;      // parameter sets
    const uint8_t *pid;          // per-cell parameter set, or null (set 0)
    const uint8_t *flags;        // per-cell gate flags
-   double *state;               // [19][stride], or 64-cell tiles (blk_index) for the BLK
-                                // kernel and k_tt06_gate_f64
-   int64_t stride;
+   double *state;               // 19 states in 64-cell tiles (blk_index)
    int n;
    const int *index;            // reaction-local -> array index, or null (identity)
    const double *Vm;
@@ -35,12 +33,10 @@ struct ReactionArgs {
 
 struct StencilArgs {
    int n;
-   int64_t stride;
    const uint32_t *bi;          // box index per cell
    const double *box;           // Vm in the padded box
-   const double *A0;            // centre weight (double)
-   const float *Aw;             // off-centre weights, NodeLocation order 1..18: [18][stride],
-                                // or 64-cell tiles (blk_index) for the BLK kernel
+   const float *Aw;             // off-centre weights, NodeLocation order 1..18, in 64-cell
+                                // tiles (blk_index); the centre weight is minus their sum
    int off[19];                 // NodeLocation offsets in the box
    double scale;                // diffusionScale
    double *dVm;
@@ -159,11 +155,9 @@ __device__ __forceinline__ void range_report(RangeRec *r, uint64_t mask, int lan
 }
 
 // kernels (kernels_f64.hip)
-template <bool BLK>
 __global__ void k_tt06_calc_f64(ReactionArgs a);
 __global__ void k_tt06_gate_f64(ReactionArgs a);
 __global__ void k_scatter_f64(int n, const uint32_t *bi, const double *src, double *box);
-template <bool BLK>
 __global__ void k_fgr_calc_f64(StencilArgs a);
 __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR,
                                 const double *dVmD, const double *iStim);

@@ -17,15 +17,9 @@
 
 namespace cdh {
 
-// state / weight addressing: BLK = the engine's 64-cell blocks, else plain
-// SoA with a row stride (the drop-in objects)
-template <bool BLK>
-__device__ __forceinline__ int64_t sidx(int q, int64_t i, int nq, int64_t stride)
-{
-   return BLK ? blk_index(q, i, nq) : q * stride + i;
-}
-
-template <bool BLK>
+// The drop-in objects keep their states and weights in 64-cell tiles:
+// field q of cell i at blk_index(q, i, nq), a wavefront's fields one
+// contiguous run.
 __global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -34,19 +28,16 @@ __global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
    const double V = a.Vm[e];
    double st[N_STATE];
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) st[s] = a.state[sidx<BLK>(s, i, N_STATE, a.stride)];
+   for (int s = 0; s < N_STATE; s++) st[s] = a.state[blk_index(s, i, N_STATE)];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
    const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
    gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) a.state[sidx<BLK>(s, i, N_STATE, a.stride)] = st[s];
+   for (int s = 0; s < N_STATE; s++) a.state[blk_index(s, i, N_STATE)] = st[s];
    a.dVm[e] = dVR;
 }
-template __global__ void k_tt06_calc_f64<false>(ReactionArgs);
-template __global__ void k_tt06_calc_f64<true>(ReactionArgs);
 
-// Gate update only (Reaction::updateGate of the pdr loop): the 12 gates at
-// Vm; states in 64-cell tiles.
+// Gate update only (Reaction::updateGate of the pdr loop): the 12 gates at Vm.
 __global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -72,7 +63,6 @@ __global__ __launch_bounds__(256) void k_scatter_f64(int n, const uint32_t *__re
 
 // Blocks are dealt round-robin to the 8 XCDs; each XCD takes a contiguous
 // range of cells, so that its L2 holds the box rows and planes they read.
-template <bool BLK>
 __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
 {
    const int xcd = blockIdx.x & 7;
@@ -85,7 +75,7 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
    double A0 = 0.0;
 #pragma unroll
    for (int k = 0; k < 18; k++) {
-      w[k] = __builtin_nontemporal_load(a.Aw + sidx<BLK>(k, i, 18, a.stride));
+      w[k] = __builtin_nontemporal_load(a.Aw + blk_index(k, i, 18));
       A0 -= w[k];
    }
    const double *phi = a.box + a.bi[i];
@@ -94,8 +84,6 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
    for (int k = 1; k < 19; k++) tmp += (double)w[k - 1] * phi[a.off[k]];
    __builtin_nontemporal_store(tmp * a.scale, a.dVm + i);
 }
-template __global__ void k_fgr_calc_f64<false>(StencilArgs);
-template __global__ void k_fgr_calc_f64<true>(StencilArgs);
 
 __global__ __launch_bounds__(256) void k_integrate_f64(int n, double dt, double *__restrict__ Vm,
                                                        const double *__restrict__ dVmR,

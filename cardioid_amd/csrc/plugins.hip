@@ -79,7 +79,6 @@ ReactionArgs reaction_args(cdh_tt06 *r, double dt, const int *index, int n, cons
    a.pid = nullptr;
    a.flags = flags;
    a.state = r->state.p;
-   a.stride = r->stride;
    a.n = n;
    a.index = index;
    a.Vm = Vm;
@@ -165,7 +164,7 @@ int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const do
       CDH_REQUIRE(n == r->n, "cdh_tt06_calc: n differs from numPoints");
       if (n == 0) return;
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVm, r->flags.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -178,7 +177,7 @@ int cdh_tt06_update_nongate(cdh_tt06 *r, double dt, const int *indexArray, int n
       if (n == 0) return;
       // the calc kernel with every gate held fixed is the non-gate update
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVR, r->flagsAllFixed.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -344,10 +343,8 @@ int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
       }
       StencilArgs &a = d->args;
       a.n = (int)p->nLocal;
-      a.stride = d->stride;
       a.bi = d->bi.p;
       a.box = d->box.p;
-      a.A0 = d->A0.p;
       a.Aw = d->Aw.p;
       for (int k = 0; k < 19; k++) a.off[k] = (int)off[k];
       a.scale = d->scale;
@@ -393,7 +390,7 @@ int cdh_fgr_calc(cdh_fgr *d, double *dVm, void *stream)
       if (d->nLocal == 0) return;
       StencilArgs a = d->args;
       a.dVm = dVm;
-      hipLaunchKernelGGL(k_fgr_calc_f64<true>, dim3(8 * a.groups_per_xcd), dim3(kBlock), 0,
+      hipLaunchKernelGGL(k_fgr_calc_f64, dim3(8 * a.groups_per_xcd), dim3(kBlock), 0,
                          S(stream), a);
       CDH_LAUNCH_CHECK();
    });

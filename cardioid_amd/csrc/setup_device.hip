@@ -128,39 +128,39 @@ __global__ __launch_bounds__(kBlock) void k_engine_weights(const uint8_t *tissue
    }
 }
 
-// one thread per (tile, neighbour pair): the two neighbours' references
-// (mean of the tile's nonzero tissue-cell weights, scaled by up and rounded
-// to FP16, summed in cell order) and the 64 cells' FP16 differences
+// one wavefront per (tile, neighbour pair), one cell per lane: the two
+// neighbours' references (mean of the tile's nonzero tissue-cell weights,
+// summed in cell order as on the host, scaled by up and rounded to FP16) and
+// each cell's FP16 differences, two to a word
 __global__ __launch_bounds__(kBlock) void k_compact(const float *W, int64_t S, int64_t n,
                                                     int64_t ntile, double up, float *Aref,
                                                     uint32_t *Aw16)
 {
-   const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+   const int64_t g = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;   // wave-uniform
+   const int lane = threadIdx.x & 63;
    if (g >= ntile * 9) return;
    const int64_t t = g / 9;
    const int p = (int)(g % 9);
-   const int64_t c0 = t * kBlk, c1 = c0 + kBlk < n ? c0 + kBlk : n;
-   float ref[2];
+   const int64_t c0 = t * kBlk, c = c0 + lane;
+   const int live = (int)((n - c0) < kBlk ? (n - c0) : kBlk);   // tissue cells of the tile
+   uint32_t word = 0;
    for (int h = 0; h < 2; h++) {
-      const float *w = W + (int64_t)(2 * p + h) * S;
+      const float w = W[(int64_t)(2 * p + h) * S + c];
       double sum = 0.0;
-      int64_t nz = 0;
-      for (int64_t c = c0; c < c1; c++)
-         if (w[c] != 0.0f) {
-            sum += w[c];
+      int nz = 0;
+      for (int j = 0; j < live; j++) {
+         const float v = __shfl(w, j);
+         if (v != 0.0f) {
+            sum += v;
             ++nz;
          }
-      ref[h] = (float)fgr::half_rne(nz ? sum / (double)nz * up : 0.0);
-      Aref[t * 18 + 2 * p + h] = ref[h];
-   }
-   for (int64_t c = c0; c < c0 + kBlk; c++) {
-      uint32_t word = 0;
-      for (int h = 0; h < 2; h++) {
-         const double d = (double)W[(int64_t)(2 * p + h) * S + c] * up - (double)ref[h];
-         word |= (uint32_t)fgr::half_bits(fgr::half_rne(d)) << (16 * h);
       }
-      Aw16[blk_index(p, c, 9)] = word;
+      const float ref = (float)fgr::half_rne(nz ? sum / (double)nz * up : 0.0);
+      if (lane == 0) Aref[t * 18 + 2 * p + h] = ref;
+      const double d = (double)w * up - (double)ref;
+      word |= (uint32_t)fgr::half_bits(fgr::half_rne(d)) << (16 * h);
    }
+   Aw16[blk_index(p, c, 9)] = word;
 }
 
 }  // namespace
@@ -214,8 +214,8 @@ void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, 
    const double wm = (double)fmax;
    const int wexp = wm > 0.0 ? 14 - (int)std::ceil(std::log2(wm)) : 0;
    *wdelta_scale = (float)std::ldexp(1.0, -wexp);
-   hipLaunchKernelGGL(k_compact, dim3(grid_for(ntile * 9)), dim3(kBlock), 0, stream, W.p, S, n,
-                      ntile, std::ldexp(1.0, wexp), Aref, Aw16);
+   hipLaunchKernelGGL(k_compact, dim3(grid_for(ntile * 9 * kBlk)), dim3(kBlock), 0, stream, W.p,
+                      S, n, ntile, std::ldexp(1.0, wexp), Aref, Aw16);
    CDH_LAUNCH_CHECK();
    CDH_HIP(hipStreamSynchronize(stream));
 }
