@@ -313,6 +313,10 @@ def main():
     out["mixed_fused"] = {
         "value": mx["rate"], "unit": "Gcell-steps/s", "steps": mx["steps"], "ms_per_step": mms,
         "speedup_vs_fp64": mx["rate"] / f64["rate"],
+        # against the plain FP64 drop-in path: the reference's loop over the
+        # drop-in Reaction / Diffusion objects
+        "speedup_vs_fp64_dropin": (mx["rate"] / res["fp64_unfused"]["rate"]
+                                   if "fp64_unfused" in res else None),
         "launch": {"waves": mx["launch"][0], "block": mx["launch"][1],
                    "occupancy": mx["launch"][2]},
         "algorithmic_bytes_per_cell_step": MIXED_BYTES,
