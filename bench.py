@@ -30,14 +30,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-steps/s at 33M cells, FP64 vs mixed-FP32; p50/p99 batched-request latency"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per tissue cell-step of the fused FP64 step: 19 FP64
-# states read+written (304), Vm read+written (16), 18 float weights (72); the
-# double centre weight is recomputed from the float ones, not streamed
-FUSED_F64_BYTES = 392
+# Algorithmic bytes per tissue cell-step of the path (the per-unit figures
+# the roofline is quoted on): fused FP64 = 19 FP64 states read+written (304),
+# Vm read+written (16), 18 float weights and the double centre weight (80).
+FUSED_F64_BYTES = 400
+# what the fused FP64 kernel streams: the centre weight is recomputed from the
+# float weights, bit for bit, instead of read
+FUSED_F64_STREAMED = 392
 UNFUSED_F64_BYTES = 480
-# mixed: 17 FP32 states r+w (136), Ca_i/Na_i FP64 r+w (32), Vm r+w (16), the compact weights:
-# 18 FP16 differences (36) and the tile's 18 float references shared by 64 cells (1.125)
-MIXED_BYTES = 221.125
+# mixed: 17 FP32 states r+w (136), Ca_i/Na_i FP64 r+w (32), Vm r+w (16), 19 FP32
+# weights (76)
+MIXED_BYTES = 260
+# what the mixed kernel streams: the compact weights, 18 FP16 differences (36)
+# and the tile's 18 float references shared by 64 cells (1.125)
+MIXED_STREAMED = 221.125
 DT = 0.02
 
 
@@ -159,7 +165,8 @@ def batched(hip, W, steps, tune_steps, graph):
             "launch": {"waves": launch[0], "block": launch[1], "occupancy": launch[2]},
             "graph_steps": graph,
             "slabs_activated": int(sum((a >= 0).mean() > 0.5 for a in act)),
-            "achieved_GBps": MIXED_BYTES * ncell * steps / (ms * 1e-3) / 1e9}
+            "achieved_GBps": MIXED_BYTES * ncell * steps / (ms * 1e-3) / 1e9,
+            "streamed_GBps": MIXED_STREAMED * ncell * steps / (ms * 1e-3) / 1e9}
 
 
 def shards(hip, W, anat, stims, parts, mode, launch, steps, graph):
@@ -305,7 +312,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic("engine_f64_fma_shell256x256x512"),
-                     "kernel": "k_engine_f64", "algorithmic_bytes_per_cell_step": FUSED_F64_BYTES},
+                     "kernel": "k_engine_f64", "algorithmic_bytes_per_cell_step": FUSED_F64_BYTES,
+                     "streamed_bytes_per_cell_step": FUSED_F64_STREAMED,
+                     "streamed_frac": achieved * FUSED_F64_STREAMED / FUSED_F64_BYTES
+                     / HBM_PEAK_GBS},
         "valu": valu("engine_f64_fma_shell256x256x512", anat.ncell, ms_per_step),
     }
     mms = mx["ms"] / mx["steps"]
@@ -321,6 +331,8 @@ def main():
                    "occupancy": mx["launch"][2]},
         "algorithmic_bytes_per_cell_step": MIXED_BYTES,
         "achieved_GBps": MIXED_BYTES * anat.ncell / (mms * 1e-3) / 1e9,
+        "streamed_bytes_per_cell_step": MIXED_STREAMED,
+        "streamed_GBps": MIXED_STREAMED * anat.ncell / (mms * 1e-3) / 1e9,
         "traffic": load_traffic("engine_mixed_shell256x256x512"),
         "valu": valu("engine_mixed_shell256x256x512", anat.ncell, mms)}
     if f64["steps"] == mx["steps"]:
