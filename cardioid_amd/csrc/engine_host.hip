@@ -299,98 +299,108 @@ int Engine::add(const SlabHost &h)
       if (t0 >= 0) tiles_.release(t0, h.ntile);
       return -1;
    }
-   const int64_t S = h.S, c0 = t0 * kBlk;
-   Slab &s = slabs_[id];
-   s = Slab{};
-   s.live = true;
-   s.tile0 = t0;
-   s.ntile = h.ntile;
-   s.box0 = b0;
-   s.nbox = h.nbox;
-   s.param0 = p0;
-   s.nparam = (int64_t)h.psets.size();
-   s.n = h.n;
-   s.geom = h.geom;
-   s.perm = h.perm;
-   s.bi.assign(h.bi.begin(), h.bi.begin() + h.n);
-   s.K0.resize(h.n);
-   for (int64_t k = 0; k < h.n; k++) s.K0[k] = h.psets[h.meta[k] & 0xFFu].K0;
-   s.sensors = sensors_;
-   s.gnx = h.gnx;
-   s.gny = h.gny;
-   s.remote_gid = h.remote_gid;
-   s.remote_bi = h.remote_bi;
+   // a failure part way (a device allocation, an upload) gives the slab's
+   // ranges back and leaves no half-built slab behind
+   try {
+      const int64_t S = h.S, c0 = t0 * kBlk;
+      Slab &s = slabs_[id];
+      s = Slab{};
+      s.live = true;
+      s.tile0 = t0;
+      s.ntile = h.ntile;
+      s.box0 = b0;
+      s.nbox = h.nbox;
+      s.param0 = p0;
+      s.nparam = (int64_t)h.psets.size();
+      s.n = h.n;
+      s.geom = h.geom;
+      s.perm = h.perm;
+      s.bi.assign(h.bi.begin(), h.bi.begin() + h.n);
+      s.K0.resize(h.n);
+      for (int64_t k = 0; k < h.n; k++) s.K0[k] = h.psets[h.meta[k] & 0xFFu].K0;
+      s.sensors = sensors_;
+      s.gnx = h.gnx;
+      s.gny = h.gny;
+      s.remote_gid = h.remote_gid;
+      s.remote_bi = h.remote_bi;
 
-   put(psets_.p + p0, h.psets);
-   put(meta_.p + c0, h.meta);
-   std::vector<uint32_t> bi(S);
-   for (int64_t k = 0; k < S; k++) bi[k] = (uint32_t)(b0 + h.bi[k]);
-   put(bi_.p + c0, bi);
-   if (mode_ != CDH_MODE_MIXED_FUSED) {
-      put(state_.p + c0 * N_STATE, block(h.state, N_STATE, S, S));
-   } else {
-      std::vector<float> s32((size_t)kMixed32 * S);
-      std::vector<double> s64((size_t)2 * S);
-      for (int64_t k = 0; k < S; k++) {
-         for (int q = 0; q < kMixed32; q++) {
-            const int src = mixed_slot_source(q);
-            s32[q * S + k] = src < 0 ? 0.0f : (float)h.state[src * S + k];
+      put(psets_.p + p0, h.psets);
+      put(meta_.p + c0, h.meta);
+      std::vector<uint32_t> bi(S);
+      for (int64_t k = 0; k < S; k++) bi[k] = (uint32_t)(b0 + h.bi[k]);
+      put(bi_.p + c0, bi);
+      if (mode_ != CDH_MODE_MIXED_FUSED) {
+         put(state_.p + c0 * N_STATE, block(h.state, N_STATE, S, S));
+      } else {
+         std::vector<float> s32((size_t)kMixed32 * S);
+         std::vector<double> s64((size_t)2 * S);
+         for (int64_t k = 0; k < S; k++) {
+            for (int q = 0; q < kMixed32; q++) {
+               const int src = mixed_slot_source(q);
+               s32[q * S + k] = src < 0 ? 0.0f : (float)h.state[src * S + k];
+            }
+            s64[k] = h.state[S_CA_I * S + k];
+            s64[S + k] = h.state[S_NA_I * S + k];
          }
-         s64[k] = h.state[S_CA_I * S + k];
-         s64[S + k] = h.state[S_NA_I * S + k];
+         put(st32_.p + c0 * kMixed32, block(s32, kMixed32, S, S));
+         put(st64_.p + c0 * 2, block(s64, 2, S, S));
       }
-      put(st32_.p + c0 * kMixed32, block(s32, kMixed32, S, S));
-      put(st64_.p + c0 * 2, block(s64, 2, S, S));
+      float wdelta_scale = h.wdelta_scale;
+      if (h.device_weights) {
+         const bool f64 = mode_ != CDH_MODE_MIXED_FUSED;
+         device_slab_weights(stream_, h.geom, h.nbox, h.n, h.ntile, h.sig_cells.data(), bi_.p + c0,
+                             (uint32_t)b0, (int64_t)h.remote_bi.size(), h.remote_bi.data(),
+                             h.remote_sig.data(), f64 ? Aw_.p + c0 * 18 : nullptr,
+                             f64 ? nullptr : Aref_.p + t0 * 18, f64 ? nullptr : Aw16_.p + c0 * 9,
+                             &wdelta_scale);
+      } else if (mode_ != CDH_MODE_MIXED_FUSED) {
+         put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
+      } else {
+         put(Aref_.p + t0 * 18, h.Aref);
+         put(Aw16_.p + c0 * 9, block(h.Aw16, 9, S, S));
+      }
+      put(box_[0].p + b0, h.box0);
+      put(box_[1].p + b0, h.box0);
+      if (sensors_ & 1) {
+         const std::vector<double> neg(S, -1.0);
+         put(tact0_.p + c0, neg);
+         put(tact_.p + c0, neg);
+         put(trec_.p + c0, neg);
+      }
+      if (sensors_ & 2) {
+         put(apd_phase_.p + c0, std::vector<uint8_t>(S, 0));
+         put(vmax_.p + c0, std::vector<double>(S, 0.0));
+         put(t90_.p + c0, std::vector<double>(S, -1.0));
+      }
+      SlabDev d{};
+      d.time[parity_] = 0.0;
+      d.loop[parity_] = 0;
+      d.active = 1;
+      d.first_blk = (int32_t)t0;
+      for (int k = 0; k < 19; k++) d.soff[k] = (uint32_t)((h.off[k] + kBoxShift) * 8);
+      d.param_base = (int32_t)p0;
+      d.nstim = (int32_t)h.stim.size();
+      d.scale = h.diffusionScale;
+      d.threshold = h.threshold;
+      d.check_range = h.check_ranges;
+      d.wdelta_scale = wdelta_scale;
+      d.vm_min = h.vm_min;
+      d.vm_max = h.vm_max;
+      put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));
+      for (int q = 0; q < d.nstim; q++) {
+         const cdh_box_stim &b = h.stim[q];
+         d.stim[q] = PulseDev{b.t0, b.tf, b.period, b.phase, b.duration, b.amplitude};
+      }
+      slab_stim_eval(d.stim, d.nstim, 0.0, &d.stim_on[parity_], d.stim_val[parity_]);
+      put(slabdev_.p + id, std::vector<SlabDev>{d});
+      put(blk_slab_.p + t0, std::vector<uint16_t>(h.ntile, (uint16_t)id));
+   } catch (...) {
+      slabs_[id] = Slab{};
+      params_.release(p0, (int64_t)h.psets.size());
+      boxes_.release(b0, h.nbox);
+      tiles_.release(t0, h.ntile);
+      throw;
    }
-   float wdelta_scale = h.wdelta_scale;
-   if (h.device_weights) {
-      const bool f64 = mode_ != CDH_MODE_MIXED_FUSED;
-      device_slab_weights(stream_, h.geom, h.nbox, h.n, h.ntile, h.sig_cells.data(), bi_.p + c0,
-                          (uint32_t)b0, (int64_t)h.remote_bi.size(), h.remote_bi.data(),
-                          h.remote_sig.data(), f64 ? Aw_.p + c0 * 18 : nullptr,
-                          f64 ? nullptr : Aref_.p + t0 * 18, f64 ? nullptr : Aw16_.p + c0 * 9,
-                          &wdelta_scale);
-   } else if (mode_ != CDH_MODE_MIXED_FUSED) {
-      put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
-   } else {
-      put(Aref_.p + t0 * 18, h.Aref);
-      put(Aw16_.p + c0 * 9, block(h.Aw16, 9, S, S));
-   }
-   put(box_[0].p + b0, h.box0);
-   put(box_[1].p + b0, h.box0);
-   if (sensors_ & 1) {
-      const std::vector<double> neg(S, -1.0);
-      put(tact0_.p + c0, neg);
-      put(tact_.p + c0, neg);
-      put(trec_.p + c0, neg);
-   }
-   if (sensors_ & 2) {
-      put(apd_phase_.p + c0, std::vector<uint8_t>(S, 0));
-      put(vmax_.p + c0, std::vector<double>(S, 0.0));
-      put(t90_.p + c0, std::vector<double>(S, -1.0));
-   }
-   SlabDev d{};
-   d.time[parity_] = 0.0;
-   d.loop[parity_] = 0;
-   d.active = 1;
-   d.first_blk = (int32_t)t0;
-   for (int k = 0; k < 19; k++) d.soff[k] = (uint32_t)((h.off[k] + kBoxShift) * 8);
-   d.param_base = (int32_t)p0;
-   d.nstim = (int32_t)h.stim.size();
-   d.scale = h.diffusionScale;
-   d.threshold = h.threshold;
-   d.check_range = h.check_ranges;
-   d.wdelta_scale = wdelta_scale;
-   d.vm_min = h.vm_min;
-   d.vm_max = h.vm_max;
-   put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));
-   for (int q = 0; q < d.nstim; q++) {
-      const cdh_box_stim &b = h.stim[q];
-      d.stim[q] = PulseDev{b.t0, b.tf, b.period, b.phase, b.duration, b.amplitude};
-   }
-   slab_stim_eval(d.stim, d.nstim, 0.0, &d.stim_on[parity_], d.stim_val[parity_]);
-   put(slabdev_.p + id, std::vector<SlabDev>{d});
-   put(blk_slab_.p + t0, std::vector<uint16_t>(h.ntile, (uint16_t)id));
    return id;
 }
 
