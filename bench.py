@@ -100,8 +100,8 @@ def load_traffic(stem):
 
 
 def valu(stem, ncell, ms_per_step, peak_tflops=78.6, simds=1024, clock_ghz=2.4):
-    """Arithmetic beside the HBM roofline: SURVEY's 1,060 flop per cell-step
-    against the FP64 vector peak, and the VALU issue fraction from rocprof's
+    """Arithmetic beside the HBM roofline: the path's 1,060 flop per cell-step
+    (cardioid_amd/flops.py) against the FP64 vector peak, and the VALU issue fraction from rocprof's
     VALU instructions per wave (4 SIMD cycles each) at this run's step time."""
     from cardioid_amd.flops import flop_per_cell_step
     FLOP_PER_CELL_STEP = flop_per_cell_step()

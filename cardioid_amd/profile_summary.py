@@ -85,7 +85,7 @@ def main(argv=None):
     lines = [f"# {a.title}", ""]
     if a.note:
         lines += [a.note, ""]
-    lines += ["## Kernel trace", "", "Command (from the repo root, on the GPU box):", "",
+    lines += ["## Kernel trace", "", "Command (from the repo root, on the MI355X):", "",
               f"    {a.cmd_trace}", "", "| kernel | calls | total (us) | average (us) | % |",
               "|---|---|---|---|---|"]
     for r in rows[:6]:
