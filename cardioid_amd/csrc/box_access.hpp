@@ -1,8 +1,13 @@
-// box_access.hpp -- stencil neighbour loads through a buffer resource.
+// box_access.hpp -- stencil neighbour loads and state streams through
+// buffer resources.
 //
 // The cell's box offset is the VGPR offset and the neighbour's offset
 // (shifted non-negative by kBoxShift points) the SGPR offset of one buffer
 // load, so none of the 19 neighbour addresses costs a VALU instruction.
+// The mixed engine's state streams work alike: the run's base address sits
+// in the resource, the lane's offset in a VGPR and the field's offset (its
+// stride is the run width, not a constant) in an SGPR, so no field address
+// is held in VGPRs across the reaction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,6 +28,36 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(const double *box)
 __device__ __forceinline__ double box_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// a raw buffer over a wave-uniform base address (offsets below 2^32 bytes)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t stream_rsrc(const void *base)
+{
+   return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, -1, 0x00020000);
+}
+
+// non-temporal (cache policy NT) loads and stores of one field
+constexpr int kNT = 2;
+__device__ __forceinline__ double stream_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                  uint32_t soff)
+{
+   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kNT));
+}
+__device__ __forceinline__ void stream_store_f64(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                 uint32_t soff)
+{
+   typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, kNT);
+}
+__device__ __forceinline__ float stream_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                 uint32_t soff)
+{
+   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kNT));
+}
+__device__ __forceinline__ void stream_store_f32(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                 uint32_t soff)
+{
+   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, kNT);
 }
 
 }  // namespace cdh

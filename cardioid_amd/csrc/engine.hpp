@@ -51,7 +51,8 @@ struct alignas(16) SlabDev {
    double time[2];
    int32_t loop[2];
    int32_t active;              // 0: the slab's tiles are skipped
-   int32_t first_blk;           // tile that advances the clock
+   int32_t first_blk;           // the slab's first tile, which advances the clock
+   int32_t ntile;               // the slab's tiles (mixed states: run_index over ntile * 64 cells)
    uint32_t soff[19];           // (offset_k + kBoxShift) * 8, NodeLocation order
    int32_t param_base;          // parameter sets of this slab start here
    int32_t nstim;
@@ -76,8 +77,8 @@ struct EngineArgs {
    const uint16_t *blk_slab;    // per 64-cell tile: slab, or kNoSlab
    SlabDev *slabs;
    double *state;               // FP64: 19 fields in 64-cell blocks
-   float *st32;                 // mixed: 17 fields in 64-cell blocks
-   double *st64;                // mixed: Ca_i, Na_i in 64-cell blocks
+   float *st32;                 // mixed: 17 fields, per slab in runs
+   double *st64;                // mixed: Ca_i, Na_i, per slab in runs
    const float *Aw;             // FP64: 18 off-centre weights in 64-cell blocks
    const float *Aref;           // mixed: per tile, 18 reference weights
    const uint32_t *Aw16;        // mixed: per cell, FP16 differences, 9 words in 64-cell blocks

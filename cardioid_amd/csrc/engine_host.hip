@@ -106,6 +106,26 @@ std::vector<T> unblock(const std::vector<T> &b, int nq, int64_t n)
    return out;
 }
 
+// the same for the mixed engine's states: a device range of S cells in runs
+// (run_index)
+template <class T>
+std::vector<T> from_runs(const std::vector<T> &b, int nq, int64_t n, int64_t S)
+{
+   std::vector<T> out((size_t)nq * n);
+   for (int q = 0; q < nq; q++)
+      for (int64_t k = 0; k < n; k++) out[q * n + k] = b[run_index(q, k, nq, S)];
+   return out;
+}
+
+template <class T>
+std::vector<T> to_runs(const std::vector<T> &soa, int nq, int64_t S)
+{
+   std::vector<T> b((size_t)nq * S);
+   for (int q = 0; q < nq; q++)
+      for (int64_t k = 0; k < S; k++) b[run_index(q, k, nq, S)] = soa[q * S + k];
+   return b;
+}
+
 template <class T>
 std::vector<T> block(const std::vector<T> &soa, int nq, int64_t S, int64_t stride)
 {
@@ -342,8 +362,8 @@ int Engine::add(const SlabHost &h)
             s64[k] = h.state[S_CA_I * S + k];
             s64[S + k] = h.state[S_NA_I * S + k];
          }
-         put(st32_.p + c0 * kMixed32, block(s32, kMixed32, S, S));
-         put(st64_.p + c0 * 2, block(s64, 2, S, S));
+         put(st32_.p + c0 * kMixed32, to_runs(s32, kMixed32, S));
+         put(st64_.p + c0 * 2, to_runs(s64, 2, S));
       }
       float wdelta_scale = h.wdelta_scale;
       if (h.device_weights) {
@@ -377,6 +397,7 @@ int Engine::add(const SlabHost &h)
       d.loop[parity_] = 0;
       d.active = 1;
       d.first_blk = (int32_t)t0;
+      d.ntile = (int32_t)h.ntile;
       for (int k = 0; k < 19; k++) d.soff[k] = (uint32_t)((h.off[k] + kBoxShift) * 8);
       d.param_base = (int32_t)p0;
       d.nstim = (int32_t)h.stim.size();
@@ -719,8 +740,8 @@ std::vector<double> Engine::state19(int id)
    std::vector<double> b64((size_t)2 * S);
    download(st32_.p + c0 * kMixed32, b32.data(), b32.size());
    download(st64_.p + c0 * 2, b64.data(), b64.size());
-   const std::vector<float> s32 = unblock(b32, kMixed32, n);
-   const std::vector<double> s64 = unblock(b64, 2, n);
+   const std::vector<float> s32 = from_runs(b32, kMixed32, n, S);
+   const std::vector<double> s64 = from_runs(b64, 2, n, S);
    const std::vector<double> v = vm(id);
    std::vector<double> st((size_t)N_STATE * n, 0.0);
    for (int64_t k = 0; k < n; k++) {
@@ -757,12 +778,12 @@ void Engine::set_state19(int id, const std::vector<double> &st, const std::vecto
       for (int64_t k = 0; k < n; k++) {
          for (int q = 0; q < kMixed32; q++) {
             const int d = mixed_slot_source(q);
-            if (d >= 0) b32[blk_index(q, k, kMixed32)] = (float)st[d * n + k];
+            if (d >= 0) b32[run_index(q, k, kMixed32, S)] = (float)st[d * n + k];
          }
          const double K_i = K_.c9 * (st[S_DVK_I * n + k] - v[k]);
-         b32[blk_index(4, k, kMixed32)] = (float)(K_i - s.K0[k]);
-         b64[blk_index(0, k, 2)] = st[S_CA_I * n + k];
-         b64[blk_index(1, k, 2)] = st[S_NA_I * n + k];
+         b32[run_index(4, k, kMixed32, S)] = (float)(K_i - s.K0[k]);
+         b64[run_index(0, k, 2, S)] = st[S_CA_I * n + k];
+         b64[run_index(1, k, 2, S)] = st[S_NA_I * n + k];
       }
       put(st32_.p + c0 * kMixed32, b32);
       put(st64_.p + c0 * 2, b64);

@@ -70,16 +70,25 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    }
    const double dVD = (double)acc * (v.scale * (double)S.wdelta_scale);
 
-   // states stream once per step (non-temporal); the tile's base addresses
-   // are wave-uniform
-   float *p32 = a.st32 + (int64_t)blk * (N_STATE32 * kBlk) + lane;
-   double *p64 = a.st64 + (int64_t)blk * (N_STATE64 * kBlk) + lane;
+   // states stream once per step (non-temporal), in the slab's runs of kRun
+   // cells (run_index): the base addresses and the run width are wave-uniform
+   // (buffer resources over the run's base, field s at SGPR offset
+   // s * run width * field size, the lane's cell at VGPR offset)
+   const int first = __builtin_amdgcn_readfirstlane(S.first_blk);
+   const int64_t k0 = (int64_t)(blk - first) * kBlk;
+   const int64_t r0 = k0 & ~(kRun - 1);
+   const uint32_t rw =
+       (uint32_t)run_width(k0, (int64_t)__builtin_amdgcn_readfirstlane(S.ntile) * kBlk);
+   const int64_t c0 = (int64_t)first * kBlk + r0;
+   const __amdgpu_buffer_rsrc_t R32 = stream_rsrc(a.st32 + c0 * N_STATE32);
+   const __amdgpu_buffer_rsrc_t R64 = stream_rsrc(a.st64 + c0 * N_STATE64);
+   const uint32_t lc = (uint32_t)(k0 - r0 + lane);
    float s32[N_STATE32];
 #pragma unroll
-   for (int s = 0; s < N_STATE32; s++) s32[s] = __builtin_nontemporal_load(p32 + kBlk * s);
+   for (int s = 0; s < N_STATE32; s++) s32[s] = stream_load_f32(R32, 4u * lc, 4u * rw * s);
    double s64[N_STATE64];
 #pragma unroll
-   for (int s = 0; s < N_STATE64; s++) s64[s] = __builtin_nontemporal_load(p64 + kBlk * s);
+   for (int s = 0; s < N_STATE64; s++) s64[s] = stream_load_f64(R64, 8u * lc, 8u * rw * s);
    const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
    const double K_i = P.K0 + (double)s32[M_DK_I];
    double iK;
@@ -94,9 +103,9 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    // K_i follows dVK_i = K_i/c9 + Vm of the reference: dK_i = c9*dt*(iK - dVm_D - iStim)
    s32[M_DK_I] = (float)((double)s32[M_DK_I] + (a.dt * a.K->c9) * (iK - dVD - iStim));
 #pragma unroll
-   for (int s = 0; s < N_STATE32; s++) __builtin_nontemporal_store(s32[s], p32 + kBlk * s);
+   for (int s = 0; s < N_STATE32; s++) stream_store_f32(s32[s], R32, 4u * lc, 4u * rw * s);
 #pragma unroll
-   for (int s = 0; s < N_STATE64; s++) __builtin_nontemporal_store(s64[s], p64 + kBlk * s);
+   for (int s = 0; s < N_STATE64; s++) stream_store_f64(s64[s], R64, 8u * lc, 8u * rw * s);
    if (live) {
       a.box_next[b] = Vn;
       if (a.sensors) engine_sensors(a, v, i, V, Vn, P.Vm0);

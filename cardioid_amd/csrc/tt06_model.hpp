@@ -80,6 +80,28 @@ __host__ __device__ inline int64_t blk_index(int64_t q, int64_t i, int nq)
    return blk_base(i, nq) + (int64_t)kBlk * q;
 }
 
+// The mixed engine's state arrays: per slab, runs of kRun cells, field-major
+// inside a run (field q of the run's cells one contiguous stretch), the last
+// run as wide as the cells left (a multiple of 64, as slabs are).  Measured
+// on the 20.1M-cell shell on MI355X, one box, one process: the mixed step
+// takes 0.922 ms with 4096-cell runs against 0.930 ms with 64-cell blocks
+// (0.928 with 1024 or 2048, 0.932 with 8192, 0.949 with 16384): the width
+// changes how the concurrent read-and-write streams fall on the HBM channels,
+// measured rather than modelled.  The FP64 step keeps 64-cell blocks:
+// 1.446 ms against 1.460 ms with 4096-cell runs and 1.541 ms with 8192.
+constexpr int64_t kRun = 4096;
+__host__ __device__ inline int64_t run_width(int64_t k, int64_t S)
+{
+   const int64_t r0 = k & ~(kRun - 1);
+   return S - r0 < kRun ? S - r0 : kRun;
+}
+// field q of slab cell k of an nq-field array over a slab of S cells
+__host__ __device__ inline int64_t run_index(int64_t q, int64_t k, int nq, int64_t S)
+{
+   const int64_t r0 = k & ~(kRun - 1);
+   return r0 * nq + q * run_width(k, S) + (k - r0);
+}
+
 // Per-cell-type parameters (a reaction object's CellTypeParms).
 struct CellType {
    int s_switch;
