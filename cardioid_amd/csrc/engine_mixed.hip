@@ -50,25 +50,11 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    const uint32_t b = a.bi[i];
    const uint32_t b8 = b * 8u;
    const __amdgpu_buffer_rsrc_t R = box_rsrc(a.box_cur);
+   // the cell's parameter set first: every later wait for it would also
+   // wait for whatever was issued before it
+   const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
+   __builtin_amdgcn_sched_barrier(0);
    const double V = box_load(R, b8, S.soff[0]);
-
-   // the stencil first: its neighbour and weight loads are issued with the
-   // state loads at the start, so the reaction's arithmetic covers them all
-   // weight (in units of 2^-wexp) = the tile's reference (wave-uniform,
-   // scalar loads) plus the cell's FP16 difference, exact zero for a zero
-   // weight; the unit goes into the diffusion scale
-   const float *ref = a.Aref + (int64_t)blk * 18;
-   const uint32_t *wp = a.Aw16 + (int64_t)blk * (9 * kBlk) + lane;
-   float acc = 0.0f;
-#pragma unroll
-   for (int p = 0; p < 9; p++) {
-      const uint32_t h = __builtin_nontemporal_load(wp + kBlk * p);
-      const float w0 = half_lo_plus(h, ref[2 * p]);
-      const float w1 = half_hi_plus(h, ref[2 * p + 1]);
-      acc = __builtin_fmaf(w0, (float)(box_load(R, b8, S.soff[2 * p + 1]) - V), acc);
-      acc = __builtin_fmaf(w1, (float)(box_load(R, b8, S.soff[2 * p + 2]) - V), acc);
-   }
-   const double dVD = (double)acc * (v.scale * (double)S.wdelta_scale);
 
    // states stream once per step (non-temporal), in the slab's runs of kRun
    // cells (run_index): the base addresses and the run width are wave-uniform
@@ -89,13 +75,39 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    double s64[N_STATE64];
 #pragma unroll
    for (int s = 0; s < N_STATE64; s++) s64[s] = stream_load_f64(R64, 8u * lc, 8u * rw * s);
-   const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
+   __builtin_amdgcn_sched_barrier(0);
+   // then the stencil's weights and neighbours: loads return in issue order,
+   // so the reaction waits for the states only and the stencil's loads land
+   // under its arithmetic; the sum comes after the reaction
+   const uint32_t *wp = a.Aw16 + (int64_t)blk * (9 * kBlk) + lane;
+   uint32_t hw[9];
+   double nb[18];
+#pragma unroll
+   for (int p = 0; p < 9; p++) hw[p] = __builtin_nontemporal_load(wp + kBlk * p);
+#pragma unroll
+   for (int k = 0; k < 18; k++) nb[k] = box_load(R, b8, S.soff[k + 1]);
+   __builtin_amdgcn_sched_barrier(0);
+
    const double K_i = P.K0 + (double)s32[M_DK_I];
    double iK;
    double dVR = 0.0;
    iK = 0.0;
    if (a.gate_mask & (1u << 12)) dVR = nongateM(*a.K, P, a.coef, a.dt, V, K_i, s32, s64, &iK);
    gatesM(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, s32, a.gate_mask);
+
+   // the stencil: weight (in units of 2^-wexp) = the tile's reference
+   // (wave-uniform, scalar loads) plus the cell's FP16 difference, exact zero
+   // for a zero weight; the unit goes into the diffusion scale
+   const float *ref = a.Aref + (int64_t)blk * 18;
+   float acc = 0.0f;
+#pragma unroll
+   for (int p = 0; p < 9; p++) {
+      const float w0 = half_lo_plus(hw[p], ref[2 * p]);
+      const float w1 = half_hi_plus(hw[p], ref[2 * p + 1]);
+      acc = __builtin_fmaf(w0, (float)(nb[2 * p] - V), acc);
+      acc = __builtin_fmaf(w1, (float)(nb[2 * p + 1] - V), acc);
+   }
+   const double dVD = (double)acc * (v.scale * (double)S.wdelta_scale);
 
    engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
