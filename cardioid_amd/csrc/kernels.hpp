@@ -17,7 +17,9 @@ constexpr int kMaxStim = 8;
 
 // Reaction over n cells of one reaction group (drop-in object).
 struct ReactionArgs {
-   TT06Const K;
+   // the constants in device memory rather than a kernel argument: the
+   // kernel would otherwise load all of them into SGPRs at entry and spill
+   const TT06Const *K;
    const double *coef;          // 468 dt-scaled fit coefficients
    const ParamSet *params;      // parameter sets
    const uint8_t *pid;          // per-cell parameter set, or null (set 0)
@@ -28,7 +30,7 @@ struct ReactionArgs {
    const double *Vm;
    double *dVm;
    double dt;
-   uint32_t gate_mask;          // all 12 bits set; see gates64
+   uint32_t gate_mask;          // bits 0..12 set; see gates64 and k_tt06_calc_f64
 };
 
 struct StencilArgs {

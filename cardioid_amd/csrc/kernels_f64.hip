@@ -30,8 +30,12 @@ __global__ __launch_bounds__(256) void k_tt06_calc_f64(ReactionArgs a)
 #pragma unroll
    for (int s = 0; s < N_STATE; s++) st[s] = a.state[blk_index(s, i, N_STATE)];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
-   const double dVR = nongate64(a.K, P, a.coef, a.dt, V, st);
-   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
+   // the non-gate update in a basic block of its own (gate_mask bit 12,
+   // always set): otherwise the compiler hoists its scalar coefficient loads
+   // to the kernel entry and spills them
+   double dVR = 0.0;
+   if (a.gate_mask & (1u << 12)) dVR = nongate64(*a.K, P, a.coef, a.dt, V, st);
+   gates64(a.coef, a.K->exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
    for (int s = 0; s < N_STATE; s++) a.state[blk_index(s, i, N_STATE)] = st[s];
    a.dVm[e] = dVR;
@@ -48,7 +52,7 @@ __global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
 #pragma unroll
    for (int s = S_M; s < N_STATE; s++) st[s] = a.state[blk_index(s, i, N_STATE)];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
-   gates64(a.coef, a.K.exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
+   gates64(a.coef, a.K->exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
 #pragma unroll
    for (int s = S_M; s < N_STATE; s++) a.state[blk_index(s, i, N_STATE)] = st[s];
 }

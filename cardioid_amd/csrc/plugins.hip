@@ -31,6 +31,7 @@ struct TT06Reaction {
    int n = 0;
    int64_t stride = 0;
    TT06Const K;
+   DevBuf<TT06Const> Kd;            // the same constants on the device
    CellType ct;
    DevBuf<double> coef;
    DevBuf<ParamSet> params;
@@ -73,7 +74,7 @@ ReactionArgs reaction_args(cdh_tt06 *r, double dt, const int *index, int n, cons
                            double *dVm, const uint8_t *flags)
 {
    ReactionArgs a;
-   a.K = r->K;
+   a.K = r->Kd.p;
    a.coef = r->coef.p;
    a.params = r->params.p;
    a.pid = nullptr;
@@ -84,7 +85,7 @@ ReactionArgs reaction_args(cdh_tt06 *r, double dt, const int *index, int n, cons
    a.Vm = Vm;
    a.dVm = dVm;
    a.dt = dt;
-   a.gate_mask = 0xFFFu;
+   a.gate_mask = 0x1FFFu;
    return a;
 }
 
@@ -128,6 +129,8 @@ int cdh_tt06_create(const cdh_tt06_params *p, int numPoints, cdh_tt06 **out)
       std::vector<double> coef = scaled_fit_coefficients(p->fit_l, p->fit_m, p->fit_coef, p->dt);
       r->coef.alloc(coef.size());
       r->coef.upload(coef.data(), coef.size());
+      r->Kd.alloc(1);
+      r->Kd.upload(&r->K, 1);
       ParamSet ps = make_param_set(r->ct, p->s_kr, p->s_cal, p->s_na);
       r->params.alloc(1);
       r->params.upload(&ps, 1);
