@@ -119,6 +119,18 @@ def valu(stem, ncell, ms_per_step, peak_tflops=78.6, simds=1024, clock_ghz=2.4):
     return out
 
 
+def stream_ceiling(hip, cells, reps=20):
+    """The device's HBM rates for the fused FP64 kernel's state pattern (19
+    FP64 fields per cell in 64-cell blocks, one 8-byte access per lane and
+    field) over the same number of cells: read only, write only, and read
+    and written in place (cdh_stream_rate).  The 8 TB/s peak is not
+    reachable for writes; these are the practical ceilings."""
+    out = {p + "_GBps": hip.stream_rate(p, cells, reps) for p in ("read", "write", "inplace")}
+    out["pattern"] = ("19 FP64 fields per cell in 64-cell blocks, one non-temporal 8-byte access "
+                      f"per lane and field, {cells} cells, {reps} passes each")
+    return out
+
+
 def fused_run(hip, slab, ncell, mode, steps, warmup, tune_steps, graph):
     """One slab in the engine: autotuned launch, graph replay, HIP-event time."""
     cells, box, _ = slab.extent()
@@ -318,6 +330,13 @@ def main():
                      / HBM_PEAK_GBS},
         "valu": valu("engine_f64_fma_shell256x256x512", anat.ncell, ms_per_step),
     }
+    ceil = stream_ceiling(hip, anat.ncell)
+    out["stream_ceiling"] = ceil
+    streamed = achieved * FUSED_F64_STREAMED / FUSED_F64_BYTES
+    out["roofline"]["streamed_GBps"] = streamed
+    out["roofline"]["streamed_vs_inplace_stream"] = streamed / ceil["inplace_GBps"]
+    log(f"stream ceiling: read {ceil['read_GBps']:.0f}, write {ceil['write_GBps']:.0f}, in place "
+        f"{ceil['inplace_GBps']:.0f} GB/s; the FP64 step streams {streamed:.0f} GB/s")
     mms = mx["ms"] / mx["steps"]
     both = (f64["act"] >= 0) & (mx["act"] >= 0)
     out["mixed_fused"] = {

@@ -40,6 +40,7 @@ SOURCES = {
     "host/simulate.hip": ["-ffp-contract=off"],
     "checkpoint.cpp": ["-x", "hip", "-ffp-contract=off"],
     "setup_device.hip": ["-ffp-contract=off"],
+    "stream_probe.hip": ["-ffp-contract=off"],
 }
 
 

@@ -110,7 +110,7 @@ EXPORTS = [
     "cdh_checkpoint_write", "cdh_checkpoint_read", "cdh_sim_write_checkpoint",
     "cdh_sim_read_checkpoint", "cdh_fibre_conductivity", "cdh_fibre_conductivity_int",
     "cdh_fgr_weights", "cdh_batch_link_halo", "cdh_batch_write_checkpoint",
-    "cdh_batch_read_checkpoint",
+    "cdh_batch_read_checkpoint", "cdh_stream_rate",
 ]
 
 ERR_FULL = 5
@@ -218,6 +218,7 @@ def lib():
         L.cdh_fgr_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                       C.c_double, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.cdh_stream_rate.argtypes = [C.c_int, C.c_int64, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -230,6 +231,19 @@ def check(rc: int, what: str = ""):
 
 def ptr(a, t=dp):
     return a.ctypes.data_as(t)
+
+
+STREAM_PATTERNS = {"read": 0, "write": 1, "inplace": 2}
+
+
+def stream_rate(pattern: str, cells: int, reps: int = 20) -> float:
+    """GB/s of one HBM stream in the fused FP64 kernel's state pattern (19
+    FP64 fields per cell in 64-cell blocks): "read", "write" or "inplace"
+    (cdh_stream_rate)."""
+    g = C.c_double()
+    check(lib().cdh_stream_rate(STREAM_PATTERNS[pattern], int(cells), int(reps), C.byref(g)),
+          "cdh_stream_rate")
+    return g.value
 
 
 def write_checkpoint(directory: str, loop: int, time: float, box: Sequence[int], gid, Vm, state,

@@ -393,6 +393,15 @@ int cdh_batch_slab_info(cdh_batch *b, int slab, int64_t *ncell, double *time, in
 int cdh_batch_info(cdh_batch *b, int64_t *device_bytes, int64_t *cells_used, int64_t *box_used,
                    int *nslab);
 
+/* ---- measurement ----------------------------------------------------------
+ * No counterpart in the reference: the practical ceiling for the roofline of
+ * the fused FP64 step.  Times `reps` passes of one HBM stream over `cells`
+ * cells of 19 FP64 fields in 64-cell blocks, one non-temporal 8-byte access
+ * per lane and field (the fused kernel's state pattern): pattern 0 reads,
+ * 1 writes, 2 reads and writes in place.  *gbps = bytes moved / time.
+ * Allocates and frees its own buffer (cells * 152 bytes) and stream. */
+int cdh_stream_rate(int pattern, int64_t cells, int reps, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,3 +132,14 @@ def test_checkpoint_of_no_cells(tmp_path):
                                 np.zeros(0), np.zeros((19, 0)))
     vm, st, loop, time = hip.read_checkpoint(path, np.zeros(0, np.int64))
     assert vm.size == 0 and st.shape == (19, 0) and (loop, time) == (3, 0.06)
+
+
+def test_stream_probe_rejects_bad_arguments():
+    g = C.c_double()
+    L = hip.lib()
+    assert L.cdh_stream_rate(3, 1024, 1, C.byref(g)) == 1
+    assert L.cdh_stream_rate(0, 0, 1, C.byref(g)) == 1
+    assert L.cdh_stream_rate(2, 1024, 0, C.byref(g)) == 1
+    assert L.cdh_stream_rate(1, 1024, 1, None) == 1
+    with pytest.raises(KeyError):
+        hip.stream_rate("copy", 1024)
