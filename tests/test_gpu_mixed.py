@@ -89,3 +89,38 @@ def test_mixed_state_roundtrip_and_drug_scales():
     assert np.array_equal(s2.vm(), vm)
     s1.close()
     s2.close()
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (65, 8, 8), (127, 8, 8)])
+def test_mixed_state_runs_at_their_edges(dims):
+    """The mixed engine keeps its states in per-slab runs of 4096 cells, the
+    last run as wide as the cells left: slabs of exactly one run (4096
+    cells), one run and a 64-cell remainder (4160), and one run and a 4032-
+    cell remainder (8128) activate within one time step of FP64, and a state
+    exported and imported mid-run continues as the original does.  The
+    import goes to a simulation without the stimulus: the original's pulse
+    ended at 1 ms, and the new one's clock starts at 0."""
+    nx, ny, nz = dims
+    say(f"mixed state runs: brick {nx}x{ny}x{nz} ({nx * ny * nz} cells)")
+    a = W.brick(nx, ny, nz)
+    stim = [W.box_stimulus(xMax=3, nx=nx, ny=ny, nz=nz)]
+    nsteps = 40 * nx
+    ref = run(a, stim, "fp64_fused", nsteps)
+    mix = run(a, stim, "mixed_fused", nsteps)
+    dact = np.abs(mix["act"] - ref["act"])
+    say(f"  activated {np.mean(ref['act'] >= 0):.3f}, activation max diff {dact.max():.4f} ms")
+    assert (ref["act"] >= 0).all() and (mix["act"] >= 0).all()
+    assert dact.max() <= DT * 1.0000001
+    s1 = hip.Simulation(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stim, mode="mixed_fused")
+    s1.run(100)
+    s2 = hip.Simulation(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, [], mode="mixed_fused")
+    s2.set_state(s1.state(), s1.vm())
+    s1.run(100)
+    s2.run(100)
+    dv = np.abs(s1.vm() - s2.vm()).max()
+    say(f"  continued after import: max |dVm| {dv:.3e} mV")
+    # K_i is held as an FP32 offset that the dVK_i field round-trips through FP64
+    assert dv <= 1e-6
+    np.testing.assert_allclose(s2.state(), s1.state(), rtol=1e-5, atol=1e-9)
+    s1.close()
+    s2.close()
