@@ -172,10 +172,45 @@ void Engine::track(DevBuf<T> &b, size_t count)
    bytes_ += (int64_t)(count * sizeof(T));
 }
 
+char *Engine::pinned(size_t bytes, hipEvent_t *ev)
+{
+   PinBlock *best = nullptr;
+   for (PinBlock &b : pins_) {
+      if (b.busy && hipEventQuery(b.ev) == hipSuccess) b.busy = false;
+      if (!b.busy && b.cap >= bytes && (!best || b.cap < best->cap)) best = &b;
+   }
+   if (!best) {
+      const size_t mib = size_t(1) << 20;
+      const size_t cap = (bytes + mib - 1) / mib * mib;
+      if (pinned_bytes_ + cap > kPinnedLimit) return nullptr;
+      PinBlock b;
+      b.cap = cap;
+      CDH_HIP(hipHostMalloc((void **)&b.p, cap, hipHostMallocDefault));
+      if (hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
+         (void)hipHostFree(b.p);
+         throw Error(kErrHip, "engine: hipEventCreateWithFlags failed");
+      }
+      pins_.push_back(b);
+      pinned_bytes_ += cap;
+      best = &pins_.back();
+   }
+   best->busy = true;
+   *ev = best->ev;
+   return best->p;
+}
+
 template <class T>
 void Engine::put(T *dst, const std::vector<T> &src)
 {
    if (src.empty()) return;
+   const size_t bytes = src.size() * sizeof(T);
+   hipEvent_t ev = nullptr;
+   if (char *p = pinned(bytes, &ev)) {
+      std::memcpy(p, src.data(), bytes);
+      CDH_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, stream_));
+      CDH_HIP(hipEventRecord(ev, stream_));
+      return;
+   }
    auto keep = std::make_shared<std::vector<T>>(src);
    CDH_HIP(hipMemcpyAsync(dst, keep->data(), keep->size() * sizeof(T), hipMemcpyHostToDevice,
                           stream_));
@@ -269,6 +304,10 @@ Engine::~Engine()
    if (ev0_) (void)hipEventDestroy(ev0_);
    if (ev1_) (void)hipEventDestroy(ev1_);
    if (staging_ev_) (void)hipEventDestroy(staging_ev_);
+   for (PinBlock &b : pins_) {
+      (void)hipHostFree(b.p);
+      (void)hipEventDestroy(b.ev);
+   }
    if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -365,14 +404,9 @@ int Engine::add(const SlabHost &h)
          put(st32_.p + c0 * kMixed32, to_runs(s32, kMixed32, S));
          put(st64_.p + c0 * 2, to_runs(s64, 2, S));
       }
-      float wdelta_scale = h.wdelta_scale;
       if (h.device_weights) {
-         const bool f64 = mode_ != CDH_MODE_MIXED_FUSED;
-         device_slab_weights(stream_, h.geom, h.nbox, h.n, h.ntile, h.sig_cells.data(), bi_.p + c0,
-                             (uint32_t)b0, (int64_t)h.remote_bi.size(), h.remote_bi.data(),
-                             h.remote_sig.data(), f64 ? Aw_.p + c0 * 18 : nullptr,
-                             f64 ? nullptr : Aref_.p + t0 * 18, f64 ? nullptr : Aw16_.p + c0 * 9,
-                             &wdelta_scale);
+         // the weights are built below, after the slab's descriptor, which
+         // the set-up writes the compact weights' scale into
       } else if (mode_ != CDH_MODE_MIXED_FUSED) {
          put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
       } else {
@@ -404,7 +438,7 @@ int Engine::add(const SlabHost &h)
       d.scale = h.diffusionScale;
       d.threshold = h.threshold;
       d.check_range = h.check_ranges;
-      d.wdelta_scale = wdelta_scale;
+      d.wdelta_scale = h.wdelta_scale;
       d.vm_min = h.vm_min;
       d.vm_max = h.vm_max;
       put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));
@@ -414,6 +448,21 @@ int Engine::add(const SlabHost &h)
       }
       slab_stim_eval(d.stim, d.nstim, 0.0, &d.stim_on[parity_], d.stim_val[parity_]);
       put(slabdev_.p + id, std::vector<SlabDev>{d});
+      if (h.device_weights) {
+         // everything in stream order: no wait for the device
+         const bool f64 = mode_ != CDH_MODE_MIXED_FUSED;
+         const int64_t nremote = (int64_t)h.remote_bi.size();
+         setup_.reserve(h.nbox, h.n, nremote, S, !f64);
+         put(setup_.sigc.p, h.sig_cells);
+         if (nremote > 0) {
+            put(setup_.rbi.p, h.remote_bi);
+            put(setup_.rsig.p, h.remote_sig);
+         }
+         device_slab_weights(stream_, setup_, h.geom, h.nbox, h.n, h.ntile, bi_.p + c0,
+                             (uint32_t)b0, nremote, f64 ? Aw_.p + c0 * 18 : nullptr,
+                             f64 ? nullptr : Aref_.p + t0 * 18, f64 ? nullptr : Aw16_.p + c0 * 9,
+                             &slabdev_.p[id].wdelta_scale);
+      }
       put(blk_slab_.p + t0, std::vector<uint16_t>(h.ntile, (uint16_t)id));
    } catch (...) {
       slabs_[id] = Slab{};

@@ -16,6 +16,7 @@
 
 #include "engine.hpp"
 #include "hip_util.hpp"
+#include "setup_device.hpp"
 #include "slab.hpp"
 
 namespace cdh {
@@ -119,8 +120,11 @@ class Engine {
    void drop_graph();
    EngineArgs args(int parity) const;
    void retire_staging();
+   // async H2D: through a pinned staging block (the host returns at once,
+   // also while the stream is busy), else through a kept pageable copy
    template <class T>
-   void put(T *dst, const std::vector<T> &src);           // async H2D via kept staging
+   void put(T *dst, const std::vector<T> &src);
+   char *pinned(size_t bytes, hipEvent_t *ev);
    template <class T>
    void track(DevBuf<T> &b, size_t count);
    void check_slab(int id) const;
@@ -154,6 +158,18 @@ class Engine {
    std::vector<Slab> slabs_;
    std::vector<std::shared_ptr<void>> staging_;
    hipEvent_t staging_ev_ = nullptr;
+   // pinned staging blocks, reused once the copy recorded on `ev` is done;
+   // at most kPinnedLimit bytes in all
+   struct PinBlock {
+      char *p = nullptr;
+      size_t cap = 0;
+      hipEvent_t ev = nullptr;
+      bool busy = false;
+   };
+   static constexpr size_t kPinnedLimit = size_t(1) << 30;
+   std::vector<PinBlock> pins_;
+   size_t pinned_bytes_ = 0;
+   SlabSetupScratch setup_;        // device set-up buffers, kept across slabs
 
    DevBuf<TT06Const> dK_;
    DevBuf<double> coef_;

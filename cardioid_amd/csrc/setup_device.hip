@@ -133,9 +133,10 @@ __global__ __launch_bounds__(kBlock) void k_engine_weights(const uint8_t *tissue
 // summed in cell order as on the host, scaled by up and rounded to FP16) and
 // each cell's FP16 differences, two to a word
 __global__ __launch_bounds__(kBlock) void k_compact(const float *W, int64_t S, int64_t n,
-                                                    int64_t ntile, double up, float *Aref,
-                                                    uint32_t *Aw16)
+                                                    int64_t ntile, const double *upp,
+                                                    float *Aref, uint32_t *Aw16)
 {
+   const double up = *upp;
    const int64_t g = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;   // wave-uniform
    const int lane = threadIdx.x & 63;
    if (g >= ntile * 9) return;
@@ -165,59 +166,71 @@ __global__ __launch_bounds__(kBlock) void k_compact(const float *W, int64_t S, i
 
 }  // namespace
 
-void device_slab_weights(hipStream_t stream, const BoxGeom &geom, int64_t nbox, int64_t n,
-                         int64_t ntile, const double *sig_cells, const uint32_t *bi, uint32_t b0,
-                         int64_t nremote, const uint32_t *remote_bi, const double *remote_sig,
-                         float *Aw, float *Aref, uint32_t *Aw16, float *wdelta_scale)
+// the compaction's exponent from the slab's largest |weight|: 2^-wexp into
+// the slab's descriptor and 2^wexp for k_compact (one thread)
+__global__ void k_weight_scale(const unsigned *wmax, float *wdelta_scale, double *up)
+{
+   const int wexp = compact_weight_exponent(__uint_as_float(*wmax));
+   *wdelta_scale = (float)ldexp(1.0, -wexp);
+   *up = ldexp(1.0, wexp);
+}
+
+template <class T>
+static void grow(DevBuf<T> &b, int64_t n)
+{
+   if ((int64_t)b.n < n) b.alloc((size_t)n);
+}
+
+void SlabSetupScratch::reserve(int64_t nbox, int64_t n, int64_t nremote, int64_t S, bool compact)
+{
+   grow(tissue, nbox);
+   grow(sigbox, 6 * nbox);
+   grow(sigc, 6 * std::max<int64_t>(n, 1));
+   grow(rbi, std::max<int64_t>(nremote, 1));
+   grow(rsig, 6 * std::max<int64_t>(nremote, 1));
+   if (compact) {
+      grow(W, 18 * S);
+      grow(wmax, 1);
+      grow(up, 1);
+   }
+}
+
+void device_slab_weights(hipStream_t stream, SlabSetupScratch &sc, const BoxGeom &geom,
+                         int64_t nbox, int64_t n, int64_t ntile, const uint32_t *bi, uint32_t b0,
+                         int64_t nremote, float *Aw, float *Aref, uint32_t *Aw16,
+                         float *wdelta_scale)
 {
    const int64_t S = ntile * kBlk;
-   DevBuf<uint8_t> tissue(nbox);
-   DevBuf<double> sigbox(6 * (size_t)nbox), sigc(6 * (size_t)n);
-   CDH_HIP(hipMemsetAsync(tissue.p, 0, nbox, stream));
-   CDH_HIP(hipMemsetAsync(sigbox.p, 0, sizeof(double) * 6 * nbox, stream));
-   CDH_HIP(hipMemcpyAsync(sigc.p, sig_cells, sizeof(double) * 6 * n, hipMemcpyHostToDevice,
-                          stream));
+   CDH_REQUIRE((int64_t)sc.tissue.n >= nbox && (int64_t)sc.sigc.n >= 6 * n &&
+                   (Aw || (int64_t)sc.W.n >= 18 * S),
+               "device_slab_weights: scratch not reserved");
+   CDH_HIP(hipMemsetAsync(sc.tissue.p, 0, nbox, stream));
+   CDH_HIP(hipMemsetAsync(sc.sigbox.p, 0, sizeof(double) * 6 * nbox, stream));
    hipLaunchKernelGGL(k_scatter_box, dim3(grid_for(n)), dim3(kBlock), 0, stream, n, bi, b0,
-                      sigc.p, sigbox.p, tissue.p);
+                      sc.sigc.p, sc.sigbox.p, sc.tissue.p);
    CDH_LAUNCH_CHECK();
-   DevBuf<uint32_t> rbi(nremote);
-   DevBuf<double> rsig(6 * (size_t)nremote);
    if (nremote > 0) {
-      CDH_HIP(hipMemcpyAsync(rbi.p, remote_bi, sizeof(uint32_t) * nremote, hipMemcpyHostToDevice,
-                             stream));
-      CDH_HIP(hipMemcpyAsync(rsig.p, remote_sig, sizeof(double) * 6 * nremote,
-                             hipMemcpyHostToDevice, stream));
       hipLaunchKernelGGL(k_scatter_box, dim3(grid_for(nremote)), dim3(kBlock), 0, stream, nremote,
-                         rbi.p, 0u, rsig.p, sigbox.p, tissue.p);
+                         sc.rbi.p, 0u, sc.rsig.p, sc.sigbox.p, sc.tissue.p);
       CDH_LAUNCH_CHECK();
    }
    const fgr::WeightGeom G = fgr::weight_geom(geom);
    if (Aw) {
-      hipLaunchKernelGGL(k_engine_weights, dim3(grid_for(S)), dim3(kBlock), 0, stream, tissue.p,
-                         sigbox.p, S, bi, b0, G, Aw, (float *)nullptr, (unsigned *)nullptr);
+      hipLaunchKernelGGL(k_engine_weights, dim3(grid_for(S)), dim3(kBlock), 0, stream, sc.tissue.p,
+                         sc.sigbox.p, S, bi, b0, G, Aw, (float *)nullptr, (unsigned *)nullptr);
       CDH_LAUNCH_CHECK();
-      CDH_HIP(hipStreamSynchronize(stream));
       return;
    }
-   DevBuf<float> W(18 * (size_t)S);
-   DevBuf<unsigned> wmax(1);
-   CDH_HIP(hipMemsetAsync(wmax.p, 0, sizeof(unsigned), stream));
-   hipLaunchKernelGGL(k_engine_weights, dim3(grid_for(S)), dim3(kBlock), 0, stream, tissue.p,
-                      sigbox.p, S, bi, b0, G, (float *)nullptr, W.p, wmax.p);
+   CDH_HIP(hipMemsetAsync(sc.wmax.p, 0, sizeof(unsigned), stream));
+   hipLaunchKernelGGL(k_engine_weights, dim3(grid_for(S)), dim3(kBlock), 0, stream, sc.tissue.p,
+                      sc.sigbox.p, S, bi, b0, G, (float *)nullptr, sc.W.p, sc.wmax.p);
    CDH_LAUNCH_CHECK();
-   unsigned bits = 0;
-   CDH_HIP(hipMemcpyAsync(&bits, wmax.p, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-   CDH_HIP(hipStreamSynchronize(stream));
-   float fmax;
-   std::memcpy(&fmax, &bits, 4);
-   // the exponent rule of slab.cpp's compact_weights
-   const double wm = (double)fmax;
-   const int wexp = wm > 0.0 ? 14 - (int)std::ceil(std::log2(wm)) : 0;
-   *wdelta_scale = (float)std::ldexp(1.0, -wexp);
-   hipLaunchKernelGGL(k_compact, dim3(grid_for(ntile * 9 * kBlk)), dim3(kBlock), 0, stream, W.p,
-                      S, n, ntile, std::ldexp(1.0, wexp), Aref, Aw16);
+   hipLaunchKernelGGL(k_weight_scale, dim3(1), dim3(1), 0, stream, sc.wmax.p, wdelta_scale,
+                      sc.up.p);
    CDH_LAUNCH_CHECK();
-   CDH_HIP(hipStreamSynchronize(stream));
+   hipLaunchKernelGGL(k_compact, dim3(grid_for(ntile * 9 * kBlk)), dim3(kBlock), 0, stream, sc.W.p,
+                      S, n, ntile, sc.up.p, Aref, Aw16);
+   CDH_LAUNCH_CHECK();
 }
 
 void device_fgr_weights(hipStream_t stream, const uint8_t *tissue, const double *sigma,

@@ -11,6 +11,7 @@
 #include "engine.hpp"
 #include "fgr_stencil.hpp"
 #include "hip_util.hpp"
+#include "setup_device.hpp"
 #include "setup.hpp"
 
 namespace cdh {
@@ -55,8 +56,7 @@ void compact_weights(SlabHost &h)
    const int64_t S = h.S;
    double wmax = 0.0;
    for (float w : h.Aw) wmax = std::max(wmax, (double)std::fabs(w));
-   int wexp = 0;
-   if (wmax > 0.0) wexp = 14 - (int)std::ceil(std::log2(wmax));
+   const int wexp = compact_weight_exponent((float)wmax);   // wmax is a float
    const double up = std::ldexp(1.0, wexp);
    h.wdelta_scale = (float)std::ldexp(1.0, -wexp);
    h.Aref.assign((size_t)h.ntile * 18, 0.0f);
