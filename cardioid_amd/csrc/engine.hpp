@@ -85,7 +85,8 @@ struct EngineArgs {
    const double *box_cur;       // Vm in the slabs' padded boxes (read)
    double *box_next;            // (written)
    int parity;
-   int ntile;                   // 64-cell tiles in use
+   const uint32_t *tiles;       // the live slabs' 64-cell tiles, in arena order
+   int ntile;                   // entries of tiles
    int groups_per_xcd;          // workgroup tiles per XCD
    double dt;
    uint32_t gate_mask;          // bits 0..12 set; see gates64 and the kernels

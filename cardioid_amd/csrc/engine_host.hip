@@ -218,6 +218,22 @@ void Engine::put(T *dst, const std::vector<T> &src)
    CDH_HIP(hipEventRecord(staging_ev_, stream_));
 }
 
+void Engine::refresh_tiles()
+{
+   if (!list_dirty_) return;
+   std::vector<const Slab *> live;
+   for (const Slab &s : slabs_)
+      if (s.live) live.push_back(&s);
+   std::sort(live.begin(), live.end(),
+             [](const Slab *a, const Slab *b) { return a->tile0 < b->tile0; });
+   std::vector<uint32_t> list;
+   for (const Slab *s : live)
+      for (int64_t t = 0; t < s->ntile; t++) list.push_back((uint32_t)(s->tile0 + t));
+   nlist_ = (int64_t)list.size();
+   put(tile_list_.p, list);
+   list_dirty_ = false;
+}
+
 void Engine::retire_staging()
 {
    if (staging_.empty()) return;
@@ -266,6 +282,7 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    track(meta_, S);
    track(bi_, S);
    track(blk_slab_, ntile);
+   track(tile_list_, ntile);
    CDH_HIP(hipMemset(blk_slab_.p, 0xFF, ntile * sizeof(uint16_t)));
    track(slabdev_, cap_slabs);
    slabdev_.zero();
@@ -471,6 +488,7 @@ int Engine::add(const SlabHost &h)
       tiles_.release(t0, h.ntile);
       throw;
    }
+   list_dirty_ = true;
    return id;
 }
 
@@ -478,6 +496,7 @@ void Engine::remove(int id)
 {
    check_slab(id);
    retire_staging();
+   list_dirty_ = true;
    const size_t nl = links_.size();
    links_.erase(std::remove_if(links_.begin(), links_.end(),
                                [&](const HaloLink &l) { return l.dst == id || l.src == id; }),
@@ -511,7 +530,8 @@ EngineArgs Engine::args(int parity) const
    a.box_cur = box_[parity].p;
    a.box_next = box_[parity ^ 1].p;
    a.parity = parity;
-   a.ntile = (int)tiles_.end();
+   a.tiles = tile_list_.p;
+   a.ntile = (int)nlist_;
    const int wpg = cfg_.block / kBlk;
    const int ngroup = (a.ntile + wpg - 1) / wpg;
    a.groups_per_xcd = (ngroup + 7) / 8;
@@ -619,7 +639,7 @@ void Engine::rebuild_graph()
    hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
    (void)hipGraphDestroy(g);
    CDH_HIP(e);
-   graph_ntile_ = (int)tiles_.end();
+   graph_ntile_ = (int)nlist_;
    graph_cfg_ = cfg_;
    graph_halo_ = halo_version_;
 }
@@ -646,9 +666,10 @@ void Engine::run(int nsteps)
 {
    CDH_REQUIRE(nsteps >= 0, "engine: negative step count");
    retire_staging();
+   refresh_tiles();      // before any capture: an upload must not land in a graph
    while (nsteps > 0) {
       if (graph_steps_ > 0 && parity_ == 0 && nsteps >= graph_steps_) {
-         if (!graph_ || graph_ntile_ != (int)tiles_.end() || graph_cfg_.waves != cfg_.waves ||
+         if (!graph_ || graph_ntile_ != (int)nlist_ || graph_cfg_.waves != cfg_.waves ||
              graph_cfg_.block != cfg_.block || graph_cfg_.occ != cfg_.occ ||
              graph_halo_ != halo_version_)
             rebuild_graph();

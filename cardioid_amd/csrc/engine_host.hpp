@@ -120,6 +120,7 @@ class Engine {
    void drop_graph();
    EngineArgs args(int parity) const;
    void retire_staging();
+   void refresh_tiles();
    // async H2D: through a pinned staging block (the host returns at once,
    // also while the stream is busy), else through a kept pageable copy
    template <class T>
@@ -170,6 +171,11 @@ class Engine {
    std::vector<PinBlock> pins_;
    size_t pinned_bytes_ = 0;
    SlabSetupScratch setup_;        // device set-up buffers, kept across slabs
+   // the live slabs' tiles in arena order, the step kernel's grid: rebuilt
+   // (and uploaded in stream order) by run() after slabs come or go
+   DevBuf<uint32_t> tile_list_;
+   int64_t nlist_ = 0;
+   bool list_dirty_ = true;
 
    DevBuf<TT06Const> dK_;
    DevBuf<double> coef_;

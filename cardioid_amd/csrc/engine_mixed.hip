@@ -134,8 +134,9 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_mixed(EngineArgs a)
    const int wpg = blockDim.x >> 6;
    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
-   const int blk = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
-   if (blk >= a.ntile) return;
+   const int j = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
+   if (j >= a.ntile) return;
+   const int blk = (int)a.tiles[j];
    const int sl = a.blk_slab[blk];
    if (sl == kNoSlab) return;
    SlabDev &S = a.slabs[sl];
