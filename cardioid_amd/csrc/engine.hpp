@@ -85,8 +85,10 @@ struct EngineArgs {
    const double *box_cur;       // Vm in the slabs' padded boxes (read)
    double *box_next;            // (written)
    int parity;
-   const uint32_t *tiles;       // the live slabs' 64-cell tiles, in arena order
-   int ntile;                   // entries of tiles
+   const uint32_t *tiles;       // the live slabs' 64-cell tiles, in arena order; null when
+                                // they are the one range tile_base .. tile_base + ntile - 1
+   int tile_base;
+   int ntile;                   // live tiles
    int groups_per_xcd;          // workgroup tiles per XCD
    double dt;
    uint32_t gate_mask;          // bits 0..12 set; see gates64 and the kernels

@@ -79,8 +79,9 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 // Workgroups are dealt round-robin to the 8 XCDs; workgroup w serves a tile
 // group in the contiguous range of XCD (w mod 8), so that each XCD's L2
 // holds the box rows its cells read.  The grid is 8 * groups_per_xcd and
-// covers the live slabs' tiles only (a.tiles): tiles of removed slabs cost
-// nothing.
+// covers the live slabs' tiles only: tiles of removed slabs cost nothing.
+// When the live tiles are one range (one anatomy, a batch without holes)
+// the tile is the index plus a base, with no list to read first.
 template <int WAVES, bool FMA>
 __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
 {
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
    const int lane = threadIdx.x & 63;
    const int j = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
    if (j >= a.ntile) return;
-   const int blk = (int)a.tiles[j];
+   const int blk = a.tiles ? (int)a.tiles[j] : a.tile_base + j;
    const int sl = a.blk_slab[blk];
    if (sl == kNoSlab) return;
    SlabDev &S = a.slabs[sl];

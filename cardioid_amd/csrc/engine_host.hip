@@ -230,7 +230,9 @@ void Engine::refresh_tiles()
    for (const Slab *s : live)
       for (int64_t t = 0; t < s->ntile; t++) list.push_back((uint32_t)(s->tile0 + t));
    nlist_ = (int64_t)list.size();
-   put(tile_list_.p, list);
+   tiles_direct_ = !list.empty() && (int64_t)(list.back() - list.front()) + 1 == nlist_;
+   tile_base_ = list.empty() ? 0 : (int64_t)list.front();
+   if (!tiles_direct_) put(tile_list_.p, list);
    list_dirty_ = false;
 }
 
@@ -530,7 +532,8 @@ EngineArgs Engine::args(int parity) const
    a.box_cur = box_[parity].p;
    a.box_next = box_[parity ^ 1].p;
    a.parity = parity;
-   a.tiles = tile_list_.p;
+   a.tiles = tiles_direct_ ? nullptr : tile_list_.p;
+   a.tile_base = (int)tile_base_;
    a.ntile = (int)nlist_;
    const int wpg = cfg_.block / kBlk;
    const int ngroup = (a.ntile + wpg - 1) / wpg;
@@ -640,6 +643,8 @@ void Engine::rebuild_graph()
    (void)hipGraphDestroy(g);
    CDH_HIP(e);
    graph_ntile_ = (int)nlist_;
+   graph_direct_ = tiles_direct_;
+   graph_base_ = tile_base_;
    graph_cfg_ = cfg_;
    graph_halo_ = halo_version_;
 }
@@ -669,7 +674,8 @@ void Engine::run(int nsteps)
    refresh_tiles();      // before any capture: an upload must not land in a graph
    while (nsteps > 0) {
       if (graph_steps_ > 0 && parity_ == 0 && nsteps >= graph_steps_) {
-         if (!graph_ || graph_ntile_ != (int)nlist_ || graph_cfg_.waves != cfg_.waves ||
+         if (!graph_ || graph_ntile_ != (int)nlist_ || graph_direct_ != tiles_direct_ ||
+             graph_base_ != tile_base_ || graph_cfg_.waves != cfg_.waves ||
              graph_cfg_.block != cfg_.block || graph_cfg_.occ != cfg_.occ ||
              graph_halo_ != halo_version_)
             rebuild_graph();

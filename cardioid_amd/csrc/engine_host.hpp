@@ -176,6 +176,10 @@ class Engine {
    DevBuf<uint32_t> tile_list_;
    int64_t nlist_ = 0;
    bool list_dirty_ = true;
+   bool tiles_direct_ = false;     // the live tiles are tile_base_ .. + nlist_ - 1
+   int64_t tile_base_ = 0;
+   bool graph_direct_ = false;     // the captured graph's tile mapping
+   int64_t graph_base_ = -1;
 
    DevBuf<TT06Const> dK_;
    DevBuf<double> coef_;

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_mixed(EngineArgs a)
    const int lane = threadIdx.x & 63;
    const int j = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
    if (j >= a.ntile) return;
-   const int blk = (int)a.tiles[j];
+   const int blk = a.tiles ? (int)a.tiles[j] : a.tile_base + j;
    const int sl = a.blk_slab[blk];
    if (sl == kNoSlab) return;
    SlabDev &S = a.slabs[sl];
