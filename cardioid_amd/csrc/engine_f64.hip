@@ -62,12 +62,22 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    double dVR = 0.0;
    if (a.gate_mask & (1u << 12)) dVR = nongate64<FMA>(*a.K, P, a.coef, a.dt, V, st);
    gates64<FMA>(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, st, a.gate_mask);
+   // The states are stored once the range check and the stimulus have read
+   // the slab's descriptor (reads after a store become vector loads that
+   // wait for it) in the reference-rounding instance; the FMA instance keeps
+   // its stores first.  Measured on one MI355X: 1.532 against 1.550 ms per step
+   // with reference rounding, 1.452 against 1.434 ms with FMA.
+   if constexpr (FMA) {
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
-
+      for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
+   }
    engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);
+   if constexpr (!FMA) {
+#pragma unroll
+      for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
+   }
    if (live) {
       a.box_next[b] = Vn;
       if (a.sensors) engine_sensors(a, v, i, V, Vn, P.Vm0);

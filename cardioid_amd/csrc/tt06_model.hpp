@@ -83,7 +83,7 @@ __host__ __device__ inline int64_t blk_index(int64_t q, int64_t i, int nq)
 // The mixed engine's state arrays: per slab, runs of kRun cells, field-major
 // inside a run (field q of the run's cells one contiguous stretch), the last
 // run as wide as the cells left (a multiple of 64, as slabs are).  Measured
-// on the 20.1M-cell shell on MI355X, one box, one process: the mixed step
+// on the 20.1M-cell shell on one MI355X, in one process: the mixed step
 // takes 0.922 ms with 4096-cell runs against 0.930 ms with 64-cell blocks
 // (0.928 with 1024 or 2048, 0.932 with 8192, 0.949 with 16384): the width
 // changes how the concurrent read-and-write streams fall on the HBM channels,
