@@ -310,6 +310,33 @@ int cdh_batch_get_t90(cdh_batch *b, int slab, double *t90)
    });
 }
 
+int cdh_batch_get_sensors(cdh_batch *b, int slab, double *rows, double *time, int *loop)
+{
+   return guard([&] {
+      CDH_REQUIRE(rows && time && loop, "cdh_batch_get_sensors: null output");
+      Engine &e = engine(b);
+      const Engine::Slab &s = e.slab(slab);
+      std::vector<double> v((size_t)Engine::kSensorFields * s.n);
+      e.sensor_state(slab, v.data(), time, loop);
+      scatter(s, v.data(), rows, Engine::kSensorFields);
+   });
+}
+
+int cdh_batch_set_sensors(cdh_batch *b, int slab, const double *rows, double time, int loop)
+{
+   return guard([&] {
+      CDH_REQUIRE(rows, "cdh_batch_set_sensors: null input");
+      CDH_REQUIRE(loop >= 0, "cdh_batch_set_sensors: negative loop");
+      Engine &e = engine(b);
+      const Engine::Slab &s = e.slab(slab);
+      const int64_t n = s.n;
+      std::vector<double> v((size_t)Engine::kSensorFields * n);
+      for (int q = 0; q < Engine::kSensorFields; q++)
+         for (int64_t k = 0; k < n; k++) v[q * n + k] = rows[q * n + s.perm[k]];
+      e.set_sensor_state(slab, v.data(), time, loop);
+   });
+}
+
 int cdh_batch_range_report(cdh_batch *b, int slab, int64_t *count, int *loop, int64_t *cell,
                            double *V, double *dVmR, double *dVmD)
 {

@@ -59,6 +59,10 @@ struct alignas(16) SlabDev {
    uint32_t stim_on[2];         // bit k: stimulus k is nonzero at time[p]
    int32_t check_range;         // Vm range check on (Simulate::checkRanges)
    float wdelta_scale;          // mixed form: scale of the FP16 weight differences
+   int32_t sensor_start;        // loop of the sensors' first evaluation's step: they have
+                                // seen no Vm before it (fresh sensors, at the start or a
+                                // restart); -1 while they continue a run
+   int32_t pad0_;
    double scale;                // diffusionScale
    double threshold;            // activation threshold
    double vm_min, vm_max;       // the check's range
@@ -137,7 +141,7 @@ __device__ __forceinline__ void slab_view(const SlabDev &S, int parity, double d
 {
 #pragma clang fp contract(off)
    v.time1 = S.time[parity] + dt;
-   v.first_eval = S.loop[parity] == 0;
+   v.first_eval = S.loop[parity] == S.sensor_start;
    v.scale = S.scale;
    v.threshold = S.threshold;
    v.param_base = S.param_base;

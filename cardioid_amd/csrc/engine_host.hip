@@ -434,17 +434,7 @@ int Engine::add(const SlabHost &h)
       }
       put(box_[0].p + b0, h.box0);
       put(box_[1].p + b0, h.box0);
-      if (sensors_ & 1) {
-         const std::vector<double> neg(S, -1.0);
-         put(tact0_.p + c0, neg);
-         put(tact_.p + c0, neg);
-         put(trec_.p + c0, neg);
-      }
-      if (sensors_ & 2) {
-         put(apd_phase_.p + c0, std::vector<uint8_t>(S, 0));
-         put(vmax_.p + c0, std::vector<double>(S, 0.0));
-         put(t90_.p + c0, std::vector<double>(S, -1.0));
-      }
+      reset_sensors(s);
       SlabDev d{};
       d.time[parity_] = 0.0;
       d.loop[parity_] = 0;
@@ -458,6 +448,7 @@ int Engine::add(const SlabHost &h)
       d.threshold = h.threshold;
       d.check_range = h.check_ranges;
       d.wdelta_scale = h.wdelta_scale;
+      d.sensor_start = 0;
       d.vm_min = h.vm_min;
       d.vm_max = h.vm_max;
       put(ranges_.p + id, std::vector<RangeRec>(1, RangeRec{}));
@@ -879,20 +870,95 @@ void Engine::activation(int id, double *tact0, double *tact, double *trec)
    if (trec) download(trec_.p + c0, trec, s.n);
 }
 
-void Engine::set_clock(int id, double time, int loop)
+// fresh sensors over the slab's tiles: no activation, no APD90 yet
+void Engine::reset_sensors(const Slab &s)
 {
-   check_slab(id);
+   const int64_t c0 = s.tile0 * kBlk, S = s.ntile * kBlk;
+   if (sensors_ & 1) {
+      const std::vector<double> neg(S, -1.0);
+      put(tact0_.p + c0, neg);
+      put(tact_.p + c0, neg);
+      put(trec_.p + c0, neg);
+   }
+   if (sensors_ & 2) {
+      put(apd_phase_.p + c0, std::vector<uint8_t>(S, 0));
+      put(vmax_.p + c0, std::vector<double>(S, 0.0));
+      put(t90_.p + c0, std::vector<double>(S, -1.0));
+   }
+}
+
+void Engine::write_clock(int id, double time, int loop, int sensor_start)
+{
    CDH_REQUIRE(loop >= 0, "engine: negative loop");
    sync();
    SlabDev d{};
    download(slabdev_.p + id, &d, 1);
    d.time[parity_] = time;
    d.loop[parity_] = loop;
+   d.sensor_start = sensor_start;
    slab_stim_eval(d.stim, d.nstim, time, &d.stim_on[parity_], d.stim_val[parity_]);
    put(slabdev_.p + id, std::vector<SlabDev>{d});
    sync();
    slabs_[id].time = time;
    slabs_[id].loop = loop;
+}
+
+void Engine::set_clock(int id, double time, int loop)
+{
+   check_slab(id);
+   sync();
+   reset_sensors(slabs_[id]);
+   write_clock(id, time, loop, loop);
+}
+
+void Engine::sensor_state(int id, double *rows, double *time, int *loop)
+{
+   check_slab(id);
+   sync();
+   const Slab &s = slabs_[id];
+   const int64_t c0 = s.tile0 * kBlk, n = s.n;
+   for (int64_t k = 0; k < kSensorFields * n; k++) rows[k] = -1.0;
+   if (sensors_ & 1) {
+      download(tact0_.p + c0, rows, n);
+      download(tact_.p + c0, rows + n, n);
+      download(trec_.p + c0, rows + 2 * n, n);
+   }
+   if (sensors_ & 2) {
+      download(vmax_.p + c0, rows + 3 * n, n);
+      download(t90_.p + c0, rows + 4 * n, n);
+      std::vector<uint8_t> ph(n);
+      download(apd_phase_.p + c0, ph.data(), n);
+      for (int64_t k = 0; k < n; k++) rows[5 * n + k] = ph[k];
+   }
+   SlabDev d{};
+   download(slabdev_.p + id, &d, 1);
+   *time = d.time[parity_];
+   *loop = d.loop[parity_];
+}
+
+void Engine::set_sensor_state(int id, const double *rows, double time, int loop)
+{
+   check_slab(id);
+   sync();
+   const Slab &s = slabs_[id];
+   const int64_t c0 = s.tile0 * kBlk, n = s.n;
+   if (sensors_ & 1) {
+      put(tact0_.p + c0, std::vector<double>(rows, rows + n));
+      put(tact_.p + c0, std::vector<double>(rows + n, rows + 2 * n));
+      put(trec_.p + c0, std::vector<double>(rows + 2 * n, rows + 3 * n));
+   }
+   if (sensors_ & 2) {
+      std::vector<uint8_t> ph(n);
+      for (int64_t k = 0; k < n; k++) {
+         const double v = rows[5 * n + k];
+         CDH_REQUIRE(v == 0.0 || v == 1.0 || v == 2.0, "engine: APD phase must be 0, 1 or 2");
+         ph[k] = (uint8_t)v;
+      }
+      put(vmax_.p + c0, std::vector<double>(rows + 3 * n, rows + 4 * n));
+      put(t90_.p + c0, std::vector<double>(rows + 4 * n, rows + 5 * n));
+      put(apd_phase_.p + c0, ph);
+   }
+   write_clock(id, time, loop, -1);
 }
 
 RangeRec Engine::range_report(int id)

@@ -96,13 +96,22 @@ class Engine {
    std::vector<double> state19(int id);                   // [19][n], slab order
    void set_state19(int id, const std::vector<double> &st, const std::vector<double> &vm);
    void set_vm(int id, const std::vector<double> &vm);
-   // restart: the slab's clock (and its stimulus values) at time/loop
+   // restart (readCheckpoint and the SIMULATE object's loop and time): the
+   // slab's clock and stimulus values at time/loop, and fresh sensors, as the
+   // reference creates them anew, whose first evaluation is this step's
    void set_clock(int id, double time, int loop);
+   // the sensor arrays of a slab, slab order: kSensorFields rows of n
+   // doubles (tact0, tact, trec, vmax, t90, APD phase) with the clock
+   void sensor_state(int id, double *rows, double *time, int *loop);
+   // sets them and the clock exactly; the sensors then continue the run
+   // (the step's Vm before is their previous one)
+   void set_sensor_state(int id, const double *rows, double time, int loop);
    void activation(int id, double *tact0, double *tact, double *trec);   // slab order
    void t90(int id, double *out);
    // out-of-range record since the last call (cell as a slab-order index);
    // clears it
    RangeRec range_report(int id);
+   static constexpr int kSensorFields = 6;
 
    int mode() const { return mode_; }
    int64_t device_bytes() const { return bytes_; }
@@ -130,6 +139,8 @@ class Engine {
    void track(DevBuf<T> &b, size_t count);
    void check_slab(int id) const;
    std::vector<double> st_download_blocked(double *dev, int nq, const Slab &s);
+   void reset_sensors(const Slab &s);
+   void write_clock(int id, double time, int loop, int sensor_start);
 
    int mode_;
    double dt_;

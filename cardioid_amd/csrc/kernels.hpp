@@ -31,6 +31,7 @@ struct ReactionArgs {
    double *dVm;
    double dt;
    uint32_t gate_mask;          // bits 0..12 set; see gates64 and k_tt06_calc_f64
+   const TT06ExactConst *E;     // configuration A (exact functions) constants, or null
 };
 
 struct StencilArgs {
@@ -159,6 +160,8 @@ __device__ __forceinline__ void range_report(RangeRec *r, uint64_t mask, int lan
 // kernels (kernels_f64.hip)
 __global__ void k_tt06_calc_f64(ReactionArgs a);
 __global__ void k_tt06_gate_f64(ReactionArgs a);
+__global__ void k_tt06_calc_exact_f64(ReactionArgs a);
+__global__ void k_tt06_gate_exact_f64(ReactionArgs a);
 __global__ void k_scatter_f64(int n, const uint32_t *bi, const double *src, double *box);
 __global__ void k_fgr_calc_f64(StencilArgs a);
 __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR,

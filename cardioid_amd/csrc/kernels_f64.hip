@@ -4,6 +4,8 @@
 // reference CPU path's rounding, operation by operation.
 //
 //   k_tt06_calc_f64     TT06Dev_Reaction::calc          elec/TT06Dev_Reaction.cc:393-407
+//   k_tt06_calc_exact_f64  the same in configuration A (exact functions,
+//                       generic gates)                  elec/TT06Func.cc:64-161
 //   k_scatter_f64       FGRDiffusionOMP::updateLocalVoltage  elec/FGRDiffusionOMP.cc:75-89
 //   k_fgr_calc_f64      FGRDiffusionOMP::calc           elec/FGRDiffusionOMP.cc:109-131
 //   k_integrate_f64     Vm += dt*(dVmR+dVmD+iStim)      elec/simulationLoop.cc:221-227
@@ -13,6 +15,7 @@
 #pragma clang fp contract(off)
 
 #include "kernels.hpp"
+#include "tt06_exact.hpp"
 #include "tt06_f64.hpp"
 
 namespace cdh {
@@ -53,6 +56,44 @@ __global__ __launch_bounds__(256) void k_tt06_gate_f64(ReactionArgs a)
    for (int s = S_M; s < N_STATE; s++) st[s] = a.state[blk_index(s, i, N_STATE)];
    const ParamSet P = a.params[a.pid ? a.pid[i] : 0];
    gates64(a.coef, a.K->exp_coef, P.s_switch, a.flags[i], V, st, a.gate_mask);
+#pragma unroll
+   for (int s = S_M; s < N_STATE; s++) a.state[blk_index(s, i, N_STATE)] = st[s];
+}
+
+// Configuration A: the non-gate update with the exact fv0..fv6, then the
+// generic gate update with the exact steady states and rates, at the
+// step's Vm.  Gate flags bit 0 holds every gate of a cell (the pdr loop's
+// updateNonGate); the generic loop has no v2 loop bounds, so otherwise every
+// cell's gates advance.
+__global__ __launch_bounds__(256) void k_tt06_calc_exact_f64(ReactionArgs a)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.n) return;
+   const int e = a.index ? a.index[i] : i;
+   const double V = a.Vm[e];
+   double st[N_STATE];
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) st[s] = a.state[blk_index(s, i, N_STATE)];
+   const ParamSet P = a.params[0];
+   double dVR = 0.0;
+   if (a.gate_mask & (1u << 12)) dVR = nongate_core64(*a.K, P, ExactVoltageFns{a.E, a.K}, a.dt, V, st);
+   if (!(a.flags[i] & kFlagGatesFixed)) gates_exact64(*a.E, P.s_switch, a.dt, V, st, a.gate_mask);
+#pragma unroll
+   for (int s = 0; s < N_STATE; s++) a.state[blk_index(s, i, N_STATE)] = st[s];
+   if (a.dVm) a.dVm[e] = dVR;
+}
+
+// Configuration A gate update only (Reaction::updateGate of the pdr loop).
+__global__ __launch_bounds__(256) void k_tt06_gate_exact_f64(ReactionArgs a)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.n) return;
+   const int e = a.index ? a.index[i] : i;
+   const double V = a.Vm[e];
+   double st[N_STATE];
+#pragma unroll
+   for (int s = S_M; s < N_STATE; s++) st[s] = a.state[blk_index(s, i, N_STATE)];
+   gates_exact64(*a.E, a.params[0].s_switch, a.dt, V, st, a.gate_mask);
 #pragma unroll
    for (int s = S_M; s < N_STATE; s++) a.state[blk_index(s, i, N_STATE)] = st[s];
 }

@@ -30,8 +30,10 @@ constexpr int kBlock = 256;
 struct TT06Reaction {
    int n = 0;
    int64_t stride = 0;
+   bool exact = false;              // configuration A: tolerance 0, the exact functions
    TT06Const K;
    DevBuf<TT06Const> Kd;            // the same constants on the device
+   DevBuf<TT06ExactConst> Ed;       // configuration A's constants
    CellType ct;
    DevBuf<double> coef;
    DevBuf<ParamSet> params;
@@ -86,6 +88,7 @@ ReactionArgs reaction_args(cdh_tt06 *r, double dt, const int *index, int n, cons
    a.dVm = dVm;
    a.dt = dt;
    a.gate_mask = 0x1FFFu;
+   a.E = r->exact ? r->Ed.p : nullptr;
    return a;
 }
 
@@ -120,13 +123,29 @@ int cdh_tt06_create(const cdh_tt06_params *p, int numPoints, cdh_tt06 **out)
 {
    return guard([&] {
       CDH_REQUIRE(p && out && numPoints >= 0, "cdh_tt06_create: bad arguments");
-      check_configuration_b(p->tolerance, p->mod, p->fastGate, p->fastNonGate);
+      // tolerance 0 (the factory's default) keeps the exact functions and
+      // the generic updates whatever fastGate / fastNonGate say, as the
+      // reference sets fastReaction only for tolerance 1e-4 with mod 1
+      // (elec/TT06Dev_Reaction.cc:110-114)
+      const bool exact = p->tolerance <= 0.0;
+      if (!exact) check_configuration_b(p->tolerance, p->mod, p->fastGate, p->fastNonGate);
+      CDH_REQUIRE(p->dt > 0, "cdh_tt06_create: dt must be positive");
       auto r = std::make_unique<cdh_tt06>();
       r->n = numPoints;
       r->stride = (numPoints + 63) / 64 * 64;
+      r->exact = exact;
       tt06_constants(&r->K);
       std::memcpy(&r->ct, &p->celltype, sizeof(CellType));
-      std::vector<double> coef = scaled_fit_coefficients(p->fit_l, p->fit_m, p->fit_coef, p->dt);
+      std::vector<double> coef;
+      if (exact) {
+         TT06ExactConst E;
+         tt06_exact_constants(&E, p->mod);
+         r->Ed.alloc(1);
+         r->Ed.upload(&E, 1);
+         coef.assign(kFitCoefTotal, 0.0);
+      } else {
+         coef = scaled_fit_coefficients(p->fit_l, p->fit_m, p->fit_coef, p->dt);
+      }
       r->coef.alloc(coef.size());
       r->coef.upload(coef.data(), coef.size());
       r->Kd.alloc(1);
@@ -134,8 +153,11 @@ int cdh_tt06_create(const cdh_tt06_params *p, int numPoints, cdh_tt06 **out)
       ParamSet ps = make_param_set(r->ct, p->s_kr, p->s_cal, p->s_na);
       r->params.alloc(1);
       r->params.upload(&ps, 1);
+      // configuration B reproduces the v2 loops' per-group bounds and f-gate
+      // rounding per cell; the generic gate loop of configuration A visits
+      // every cell alike
       std::vector<uint8_t> fl(r->stride, kFlagGatesFixed), fixed(r->stride, kFlagGatesFixed);
-      for (int i = 0; i < numPoints; i++) fl[i] = gate_flags(i, numPoints);
+      for (int i = 0; i < numPoints; i++) fl[i] = exact ? 0 : gate_flags(i, numPoints);
       r->flags.alloc(r->stride);
       r->flags.upload(fl.data(), fl.size());
       r->flagsAllFixed.alloc(r->stride);
@@ -167,7 +189,8 @@ int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const do
       CDH_REQUIRE(n == r->n, "cdh_tt06_calc: n differs from numPoints");
       if (n == 0) return;
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVm, r->flags.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(r->exact ? k_tt06_calc_exact_f64 : k_tt06_calc_f64,
+                         dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -180,7 +203,8 @@ int cdh_tt06_update_nongate(cdh_tt06 *r, double dt, const int *indexArray, int n
       if (n == 0) return;
       // the calc kernel with every gate held fixed is the non-gate update
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVR, r->flagsAllFixed.p);
-      hipLaunchKernelGGL(k_tt06_calc_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(r->exact ? k_tt06_calc_exact_f64 : k_tt06_calc_f64,
+                         dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }
@@ -192,7 +216,8 @@ int cdh_tt06_update_gate(cdh_tt06 *r, double dt, const int *indexArray, int n, c
       CDH_REQUIRE(r && Vm && n == r->n, "cdh_tt06_update_gate: bad arguments");
       if (n == 0) return;
       ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, nullptr, r->flags.p);
-      hipLaunchKernelGGL(k_tt06_gate_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
+      hipLaunchKernelGGL(r->exact ? k_tt06_gate_exact_f64 : k_tt06_gate_f64,
+                         dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
       CDH_LAUNCH_CHECK();
    });
 }

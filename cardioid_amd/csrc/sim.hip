@@ -45,6 +45,7 @@ struct cdh_sim {
    std::vector<std::vector<int>> groups;             // local cells of each reaction
    int64_t unfusedBytes = 0;
    BoxGeom unfusedGeom{};
+   std::vector<double> vrest;                        // APD90's Vrest per cell (unfused)
 
    ~cdh_sim()
    {
@@ -83,6 +84,16 @@ const std::vector<int64_t> &sim_perm(cdh_sim *s)
    return s->eng->slab(s->slab).perm;
 }
 
+// The sensors, fresh: at the start, and anew at a restart, as the
+// reference creates them from the deck and checkpoints hold none
+void unfused_sensors(cdh_sim *s, host::Simulate &sim)
+{
+   sim.sensor.clear();
+   if (s->p.sensors)
+      sim.sensor.push_back(std::make_unique<host::ActivationSensor>(
+          s->n, s->p.threshold, (s->p.sensors & 2) ? s->vrest.data() : nullptr, s->stream));
+}
+
 // The reference's set-up for this path: reactions per cell type in
 // ascending local order (elec/ReactionManager.cc:184-261), the diffusion
 // (elec/diffusionFactory.cc:38-68), membrane voltage from the reactions
@@ -90,7 +101,7 @@ const std::vector<int64_t> &sim_perm(cdh_sim *s)
 // lists (elec/BoxStimulus.cc:18-34), the activation sensors.
 void create_unfused(cdh_sim *s, const cdh_sim_params *p, int nstim, const cdh_box_stim *stim)
 {
-   CDH_REQUIRE(p->gate_tail_rule,
+   CDH_REQUIRE(p->gate_tail_rule || p->exact_functions,
                "cdh_sim_create: the unfused mode is the reference's loop; its reactions always "
                "apply the v2 loop bounds (gate_tail_rule = 1)");
    const int64_t n = p->ncell;
@@ -123,8 +134,10 @@ void create_unfused(cdh_sim *s, const cdh_sim_params *p, int nstim, const cdh_bo
       if (g.empty()) continue;
       cdh_tt06_params tp{};
       tp.dt = p->dt;
-      tp.tolerance = 1e-4;
-      tp.mod = 1;
+      // configuration B (tolerance 1e-4, mod 1, v2 gates) or A (tolerance 0:
+      // the exact functions; mod selects the smoothed h and j rates)
+      tp.tolerance = p->exact_functions ? 0.0 : 1e-4;
+      tp.mod = p->exact_functions ? p->jhTauSmooth : 1;
       tp.fastGate = 1;
       tp.fastNonGate = 0;
       tp.fit_l = p->fit_l;
@@ -182,15 +195,11 @@ void create_unfused(cdh_sim *s, const cdh_sim_params *p, int nstim, const cdh_bo
       }
       sim->stimulus.push_back(std::make_unique<host::BoxStimulus>(b, cells, s->stream));
    }
-   if (p->sensors) {
-      std::vector<double> vrest;
-      if (p->sensors & 2) {
-         vrest.resize(n);
-         for (int64_t c = 0; c < n; c++) vrest[c] = p->celltypes[p->celltype[c]].Vm0;
-      }
-      sim->sensor.push_back(std::make_unique<host::ActivationSensor>(
-          n, p->threshold, (p->sensors & 2) ? vrest.data() : nullptr, s->stream));
+   if (p->sensors & 2) {
+      s->vrest.resize(n);
+      for (int64_t c = 0; c < n; c++) s->vrest[c] = p->celltypes[p->celltype[c]].Vm0;
    }
+   unfused_sensors(s, *sim);
    const BoxGeom &g = s->unfusedGeom;
    s->unfusedBytes = n * (int64_t)(N_STATE * 8 + 2 + 4 * 8 + 80 + 4 + 24) +
                      (int64_t)g.bx * g.by * g.bz * 8;
@@ -216,7 +225,12 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
                       p->mode == CDH_MODE_MIXED_FUSED || p->mode == CDH_MODE_FP64_FMA,
                   "cdh_sim_create: unknown mode");
       CDH_REQUIRE(p->dt > 0, "cdh_sim_create: dt must be positive");
-      CDH_REQUIRE(p->fit_l && p->fit_m && p->fit_coef, "cdh_sim_create: null fit arrays");
+      if (p->exact_functions && p->mode != CDH_MODE_FP64_UNFUSED)
+         throw Error(kErrUnsupported,
+                     "cdh_sim_create: the exact functions (TT06Dev tolerance 0) run in the "
+                     "unfused mode only");
+      CDH_REQUIRE(p->exact_functions || (p->fit_l && p->fit_m && p->fit_coef),
+                  "cdh_sim_create: null fit arrays");
       CDH_REQUIRE(!(p->sensors & 2) || (p->sensors & 1),
                   "cdh_sim_create: APD90 needs the activation sensors (sensors bit 0)");
       if (p->mode == CDH_MODE_MIXED_FUSED)
@@ -231,7 +245,7 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
       s->n = p->ncell;
       if (p->ncell > 0) s->gid.assign(p->gid, p->gid + p->ncell);
       tt06_constants(&s->K);
-      std::vector<double> coef = scaled_fit_coefficients(p->fit_l, p->fit_m, p->fit_coef, p->dt);
+      if (!p->exact_functions) scaled_fit_coefficients(p->fit_l, p->fit_m, p->fit_coef, p->dt);
       if (p->ncell > 0) {
          std::vector<CellType> cts(p->ncelltype);
          std::memcpy(cts.data(), p->celltypes, sizeof(CellType) * p->ncelltype);
@@ -511,8 +525,10 @@ int cdh_sim_read_checkpoint(cdh_sim *s, const char *state_file)
       s->time = time;
       if (s->eng) s->eng->set_clock(s->slab, time, loop);
       if (s->loopsim) {
+         CDH_HIP(hipStreamSynchronize(s->stream));
          s->loopsim->time = time;
          s->loopsim->loop = loop;
+         unfused_sensors(s, *s->loopsim);
       }
    });
 }

@@ -117,17 +117,36 @@ __device__ __forceinline__ double gate64(const double *__restrict__ coef, const 
 
 #define CDH_SIGM(x) ((x) / (1.0 + (x)))
 
-// Non-gate update of one cell; returns dVdt = -Iion and updates the seven
-// concentration-like states in st[].  Gates are read, not written.  FMA
-// applies to the fit polynomials only.
-template <bool FMA = false>
-__device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &P,
-                                           const double *__restrict__ coef, double dt, double Vm,
-                                           double *st)
+// The seven voltage functions of the non-gate update (fv05General /
+// fv6General, elec/TT06Func.cc:614-629) from the dt-independent rational
+// fits 0..6 (configuration B).  The exact functions of configuration A are
+// the other implementation of the same two calls (tt06_exact.hpp).
+template <bool FMA>
+struct FitVoltageFns {
+   const double *__restrict__ coef;
+   __device__ __forceinline__ void fv05(double Vm, double fv[6]) const
+   {
+      fv[0] = fit64<0, FMA>(coef, Vm);
+      fv[1] = fit64<1, FMA>(coef, Vm);
+      fv[2] = fit64<2, FMA>(coef, Vm);
+      fv[3] = fit64<3, FMA>(coef, Vm);
+      fv[4] = fit64<4, FMA>(coef, Vm);
+      fv[5] = fit64<5, FMA>(coef, Vm);
+   }
+   __device__ __forceinline__ double fv6(double dV0) const { return fit64<6, FMA>(coef, dV0); }
+};
+
+// Non-gate update of one cell (update_nonGate, elec/TT06NonGates.c:374-563);
+// returns dVdt = -Iion and updates the seven concentration-like states in
+// st[].  Gates are read, not written.  FV supplies fv0..fv6.
+template <class FV>
+__device__ __forceinline__ double nongate_core64(const TT06Const &K, const ParamSet &P,
+                                                 const FV &fvs, double dt, double Vm, double *st)
 {
-   const double fv0 = fit64<0, FMA>(coef, Vm), fv1 = fit64<1, FMA>(coef, Vm);
-   const double fv2 = fit64<2, FMA>(coef, Vm), fv3 = fit64<3, FMA>(coef, Vm);
-   const double fv4 = fit64<4, FMA>(coef, Vm), fv5 = fit64<5, FMA>(coef, Vm);
+   double fvv[6];
+   fvs.fv05(Vm, fvv);
+   const double fv0 = fvv[0], fv1 = fvv[1], fv2 = fvv[2], fv3 = fvv[3], fv4 = fvv[4];
+   const double fv5 = fvv[5];
    const double Na_i = st[S_NA_I], Ca_i = st[S_CA_I], Ca_ss = st[S_CA_SS], Ca_SR = st[S_CA_SR];
    const double fCass = st[S_FCASS], R_prime = st[S_R_PRIME];
    double dVK_i = st[S_DVK_I];
@@ -159,7 +178,7 @@ __device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &
    const double dV0 = Vm - K.c3 * log(K_i) - K.c5;
    const double dV1 = Vm - K.c3 * log(Na_i) - K.c4;
    const double dV2 = dV0 - K.c3 * log(1 + K.c2 * Na_i / K_i) + K.c5 - K.c6;
-   const double fv6 = fit64<6, FMA>(coef, dV0);
+   const double fv6 = fvs.fv6(dV0);
    const double m = st[S_M], h = st[S_H], j = st[S_J];
    const double I_NaK = c_NaK * CDH_SIGM(x0) * fv0;
    const double I_pK = c_pK * fv5 * dV0;
@@ -196,6 +215,15 @@ __device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &
    dVR += I_CaL;
    st[S_DVK_I] = dVK_i + dt * dVR;
    return dVR;
+}
+
+// configuration B: the fits (FMA applies to the fit polynomials only)
+template <bool FMA = false>
+__device__ __forceinline__ double nongate64(const TT06Const &K, const ParamSet &P,
+                                           const double *__restrict__ coef, double dt, double Vm,
+                                           double *st)
+{
+   return nongate_core64(K, P, FitVoltageFns<FMA>{coef}, dt, Vm, st);
 }
 
 // The 12 gate updates at voltage Vm (the voltage of the step's start).

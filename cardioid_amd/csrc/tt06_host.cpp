@@ -65,7 +65,83 @@ void tt06_constants(TT06Const *k)
    for (int i = 1; i < 17; i++) k->exp_coef[i] = k->exp_coef[i - 1] / i;
 }
 
+
 namespace {
+
+// 1/tau of the h and j gates with its derivative, the two branches of
+// hTauRecip / jTauRecip (elec/TT06Tau.cc:14-129): only the value and first
+// derivative at the bridge's two ends enter the cubic, in the reference's
+// operation order so that the coefficients are its doubles.
+double h_rate(double V, double *d)
+{
+   double t1, t2, d1, d2;
+   if (V < -40.0) {
+      t1 = 0.0570000 * std::exp(-(V + 80.0) / 6.80000);
+      d1 = -t1 / 6.8000;
+      t2 = 2.70000 * std::exp(0.0790000 * V) + 310000. * std::exp(0.348500 * V);
+      d2 = 0.079 * 2.70000 * std::exp(0.0790000 * V) + 0.3485 * 310000. * std::exp(0.348500 * V);
+   } else {
+      const double c = (0.770000 / 0.130000);
+      const double e = std::exp((V + 10.6600) / -11.1000);
+      const double s = 1.0 / (1.0 + e);
+      t1 = 0.0;
+      d1 = 0.0;
+      t2 = c * s;
+      d2 = c * s * s * e / 11.1;
+   }
+   *d = d1 + d2;
+   return t1 + t2;
+}
+
+double j_rate(double V, double *d)
+{
+   double t1, t2, d1, d2;
+   if (V < -40.0) {
+      const double e1 = std::exp(0.244400 * V), e2 = std::exp(-0.0439100 * V);
+      const double a1 = (-25428.0 * e1 - 6.94800e-06 * e2);
+      const double da1 = (-0.24400 * 25428.0 * e1 + 0.0439100 * 6.94800e-06 * e2);
+      const double a2 = (V + 37.7800), da2 = 1.0;
+      const double e3 = std::exp(0.311 * (V + 79.2300));
+      const double a3 = 1 / (1.0 + e3);
+      const double da3 = -a3 * a3 * e3 * 0.311;
+      t1 = a1 * a2 * a3;
+      d1 = (da1 / a1 + da2 / a2 + da3 / a3) * t1;
+      const double b1 = 0.0242400 * std::exp(-0.0105200 * V);
+      const double db1 = -0.0105200 * b1;
+      const double e4 = std::exp(-0.137800 * (V + 40.1400));
+      const double b2 = 1.0 / (1.0 + e4);
+      const double db2 = -(-0.137800) * b2 * b2 * e4;
+      t2 = b1 * b2;
+      d2 = (db1 / b1 + db2 / b2) * t2;
+   } else {
+      t1 = 0.0;
+      d1 = 0.0;
+      const double b1 = 0.6000000 * std::exp(0.0570000 * V);
+      const double db1 = 0.0570000 * b1;
+      const double e = std::exp(-0.1 * (V + 32.0000));
+      const double b2 = 1.0 / (1.0 + e);
+      const double db2 = -b2 * b2 * e * -0.1;
+      t2 = b1 * b2;
+      d2 = (db1 / b1 + db2 / b2) * t2;
+   }
+   *d = d1 + d2;
+   return t1 + t2;
+}
+
+// makeTauRecipParms + hermite (elec/TT06Tau.cc:137-157)
+void bridge(double x0, double x1, double (*rate)(double, double *), double out[6])
+{
+   double d0, d1;
+   const double p0 = rate(x0, &d0), p1 = rate(x1, &d1);
+   const double dx = x1 - x0;
+   out[0] = x0;
+   out[1] = x1;
+   out[2] = p0;
+   out[3] = d0;
+   out[4] = 3 * (p1 - p0) / (dx * dx) - (2 * d0 + d1) / dx;
+   out[5] = -2 * (p1 - p0) / (dx * dx * dx) + (d0 + d1) / (dx * dx);
+}
+
 struct Row {
    const char *name;
    int s_switch;
@@ -95,6 +171,31 @@ const Row kRows[] = {
      0.00621, 0.4712, 0.0095, 2.42e-8, 3.373e-5, 0.7888, 0.9755, 0.066, 0.999998}},
 };
 }  // namespace
+
+// The exact functions' constants (elec/TT06NonGates.c:236-248, same
+// operand order) and, for mod = 1, the h and j rate bridges
+// (TT06Func::initCnst, elec/TT06Func.cc:56-61).
+void tt06_exact_constants(TT06ExactConst *e, int mod)
+{
+   const double R = 8314.472, T = 310, F = 96485.3415;
+   const double K_o = 5.4, Na_o = 140, Ca_o = 2, g_CaL = 0.0000398, g_pK = 0.0146;
+   const double K_mk = 1, K_NaCa = 1000, K_sat = 0.1, alpha = 2.5, gamma_ = 0.35;
+   const double Km_Ca = 1.38, Km_Nai = 87.5;
+   const double c1 = F / (R * T);
+   e->f1 = c1;
+   e->f2 = -2.0 * K_o / (K_o + K_mk);
+   e->f3 = ((Km_Nai * Km_Nai * Km_Nai + Na_o * Na_o * Na_o) * (Km_Ca + Ca_o)) / (K_NaCa * Ca_o);
+   e->f4 = e->f3 * K_sat;
+   e->f5 = gamma_ * e->f1;
+   e->f6 = (Na_o * Na_o * Na_o * alpha / Ca_o);
+   e->f7a = 0.5 * g_CaL * F;
+   e->f9a = 4.0 * Ca_o * e->f7a;
+   e->f10 = g_pK;
+   bridge(-64.20, -23.3, h_rate, e->hmod);
+   bridge(-48.85, -17.6, j_rate, e->jmod);
+   e->mod = mod ? 1 : 0;
+   e->pad_ = 0;
+}
 
 bool tt06_standard_celltype(const char *name, CellType *out)
 {

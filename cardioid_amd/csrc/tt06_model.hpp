@@ -33,6 +33,18 @@ struct TT06Const {
    double exp_coef[17];   // 1/i!, the m-gate Rush-Larsen series
 };
 
+// Configuration A (tolerance 0, the exact TT06 functions): the constants of
+// fv0..fv5 (elec/TT06NonGates.c:236-248) and, for mod = 1, the cubic
+// Hermite bridges of the h and j rates, {x0, x1, c0, c1, c2, c3} with
+// 1/tau = c0 + x*(c1 + x*(c2 + x*c3)), x = V - x0, on [x0, x1]
+// (elec/TT06Tau.cc:131-157).
+struct TT06ExactConst {
+   double f1, f2, f3, f4, f5, f6, f7a, f9a, f10;
+   double hmod[6], jmod[6];
+   int mod;
+   int pad_;
+};
+
 // Fit orders (l, m) in fit order; the kernels are specialised on them.
 struct FitOrder { int l, m; };
 constexpr FitOrder kFitOrder[33] = {
@@ -128,6 +140,7 @@ inline uint8_t gate_flags(int64_t i, int64_t n)
 
 // host-side helpers (tt06_host.cpp)
 void tt06_constants(TT06Const *k);
+void tt06_exact_constants(TT06ExactConst *e, int mod);
 bool tt06_standard_celltype(const char *name, CellType *out);
 
 }  // namespace cdh

@@ -66,7 +66,8 @@ class SimParams(C.Structure):
                 ("fit_coef", dp), ("threshold", C.c_double), ("mode", C.c_int),
                 ("gate_tail_rule", C.c_int), ("sensors", C.c_int), ("variant", C.c_int),
                 ("check_ranges", C.c_int), ("vm_min", C.c_double), ("vm_max", C.c_double),
-                ("setup_on_host", C.c_int)]
+                ("setup_on_host", C.c_int), ("exact_functions", C.c_int),
+                ("jhTauSmooth", C.c_int)]
 
 
 class BatchParams(C.Structure):
@@ -110,7 +111,8 @@ EXPORTS = [
     "cdh_checkpoint_write", "cdh_checkpoint_read", "cdh_sim_write_checkpoint",
     "cdh_sim_read_checkpoint", "cdh_fibre_conductivity", "cdh_fibre_conductivity_int",
     "cdh_fgr_weights", "cdh_batch_link_halo", "cdh_batch_write_checkpoint",
-    "cdh_batch_read_checkpoint", "cdh_stream_rate",
+    "cdh_batch_read_checkpoint", "cdh_batch_get_sensors", "cdh_batch_set_sensors",
+    "cdh_stream_rate",
 ]
 
 ERR_FULL = 5
@@ -191,6 +193,8 @@ def lib():
         L.cdh_batch_write_checkpoint.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int,
                                                  C.c_char_p, C.c_int]
         L.cdh_batch_read_checkpoint.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+        L.cdh_batch_get_sensors.argtypes = [C.c_void_p, C.c_int, dp, dp, ip]
+        L.cdh_batch_set_sensors.argtypes = [C.c_void_p, C.c_int, dp, C.c_double, C.c_int]
         L.cdh_batch_run.argtypes = [C.c_void_p, C.c_int]
         L.cdh_batch_run_timed.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_sync.argtypes = [C.c_void_p]
@@ -308,7 +312,11 @@ class Simulation:
                  fits: Optional[FitSet] = None, scales: Optional[np.ndarray] = None,
                  gate_tail_rule: bool = True, sensors: int = 1, variant: int = 0,
                  check_ranges: bool = True, vm_min: float = -150.0, vm_max: float = 100.0,
-                 setup_on_host: bool = False):
+                 setup_on_host: bool = False, exact_functions: bool = False,
+                 jhTauSmooth: int = 0):
+        """exact_functions: TT06Dev configuration A (tolerance 0: the exact
+        TT06 functions, generic gate update; unfused mode only), with
+        jhTauSmooth the deck's mod keyword."""
         L = lib()
         self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
         fl, fm, fc = self.fits.packed()
@@ -332,7 +340,8 @@ class Simulation:
                       fit_l=ptr(fl, ip), fit_m=ptr(fm, ip), fit_coef=ptr(fc), threshold=threshold,
                       mode=MODES[mode], gate_tail_rule=int(gate_tail_rule), sensors=int(sensors),
                       variant=variant, check_ranges=int(check_ranges), vm_min=vm_min,
-                      vm_max=vm_max, setup_on_host=int(bool(setup_on_host)))
+                      vm_max=vm_max, setup_on_host=int(bool(setup_on_host)),
+                      exact_functions=int(bool(exact_functions)), jhTauSmooth=int(jhTauSmooth))
         st = (BoxStim * max(1, len(stimuli)))(*stimuli)
         h = C.c_void_p()
         check(L.cdh_sim_create(C.byref(p), len(stimuli), st, C.byref(h)), "cdh_sim_create")
@@ -531,6 +540,23 @@ class Batch:
         """States, Vm and clock of slab `sid` from a stateFile."""
         check(lib().cdh_batch_read_checkpoint(self.h, sid, state_file.encode()),
               "cdh_batch_read_checkpoint")
+
+    def sensors(self, sid: int):
+        """(rows (6, ncell): tact0, tact, trec, Vmax, t90, APD phase; time,
+        loop) of slab sid, the run state a checkpoint file does not hold."""
+        rows = np.zeros(6 * self.ncell[sid])
+        t, l = C.c_double(), C.c_int()
+        check(lib().cdh_batch_get_sensors(self.h, sid, ptr(rows), C.byref(t), C.byref(l)),
+              "cdh_batch_get_sensors")
+        return rows.reshape(6, -1), t.value, l.value
+
+    def set_sensors(self, sid: int, rows, time: float, loop: int):
+        """Restore what sensors() returned: the sensors then continue the run."""
+        r = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1)
+        if r.size != 6 * self.ncell[sid]:
+            raise ValueError("set_sensors: 6 rows of ncell values expected")
+        check(lib().cdh_batch_set_sensors(self.h, sid, ptr(r), float(time), int(loop)),
+              "cdh_batch_set_sensors")
 
     def link_halo(self, dst: int, src: int) -> int:
         """After every step, slab dst's remote cells owned by slab src take

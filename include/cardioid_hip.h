@@ -72,12 +72,24 @@ int cdh_celltype_standard(const char *name, cdh_celltype *out);
 /* ------------------------------------------------------------------------ */
 typedef struct cdh_tt06 cdh_tt06;
 
+/* Two configurations of the factory's keywords are implemented:
+ *   B: tolerance = 1e-4, mod = 1, fastGate >= 1, fastNonGate = 0 -- the
+ *      rational fits of the fit file, the v2 gate kernels, the generic
+ *      non-gate update;
+ *   A: tolerance <= 0 (the factory's default, TT06Dev_Reaction.cc:600-602)
+ *      -- the exact TT06 functions with the generic gate and non-gate
+ *      updates (TT06Func.cc:64-161, TT06NonGates.c:319-364); mod = 1 takes
+ *      the smoothed h and j rates (TT06Tau.cc:131-151); fastGate and
+ *      fastNonGate are ignored, as the reference ignores them there
+ *      (:110-114); the fit arrays are not read and may be NULL.
+ * Anything else returns CDH_ERR_UNSUPPORTED (e.g. fits with fastGate = 0,
+ * which applies dt twice in the reference). */
 typedef struct {
    double dt;                  /* time step the fits are scaled by (ms) */
-   double tolerance;           /* REACTION keyword; must be 1e-4 */
-   int mod;                    /* REACTION keyword (jhTauSmooth); must be 1 */
-   int fastGate;               /* must be >= 1: the v2 gate kernels */
-   int fastNonGate;            /* must be 0: the generic non-gate update */
+   double tolerance;           /* REACTION keyword: 1e-4 (B) or <= 0 (A) */
+   int mod;                    /* REACTION keyword (jhTauSmooth): 1 in B; 0 or 1 in A */
+   int fastGate;               /* B: >= 1, the v2 gate kernels */
+   int fastNonGate;            /* B: 0, the generic non-gate update */
    const int *fit_l;           /* 33 fits in the order of the fitFile's      */
    const int *fit_m;           /* `functions` list; coefficients packed, l+m */
    const double *fit_coef;     /* per fit (elec/pade.hh:4-26 layout)          */
@@ -237,6 +249,11 @@ typedef struct {
                                    initializeSimulate.cc:96-98) */
    int setup_on_host;           /* 1: FGR weights built on the host; 0 (default): on
                                    the device, bit for bit the same */
+   int exact_functions;         /* 1: TT06Dev configuration A (tolerance 0: the exact
+                                   functions, generic gate update; fit arrays may be
+                                   NULL); CDH_MODE_FP64_UNFUSED only */
+   int jhTauSmooth;             /* with exact_functions: the mod keyword (1: smoothed
+                                   h and j rates) */
 } cdh_sim_params;
 
 int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim, cdh_sim **out);
@@ -293,7 +310,10 @@ int cdh_sim_write_checkpoint(cdh_sim *s, const char *dir, int ascii, char *state
                              int path_len);
 /* sets the states and Vm from the file (fields it lacks keep their
  * values); the engine's clock takes the file's loop and time (written with
- * %f, as the reference restarts from); sensors are not part of a checkpoint */
+ * %f, as the reference restarts from).  Sensors are not part of a
+ * checkpoint: as the reference's, created anew at a restart, they start
+ * fresh, their first evaluation at the step after the restart, and record
+ * cells already above a level then. */
 int cdh_sim_read_checkpoint(cdh_sim *s, const char *state_file);
 
 
@@ -365,6 +385,16 @@ int cdh_batch_link_halo(cdh_batch *b, int dst, int src, int64_t *npoints);
 int cdh_batch_write_checkpoint(cdh_batch *b, int slab, const char *dir, int ascii,
                                char *state_path, int path_len);
 int cdh_batch_read_checkpoint(cdh_batch *b, int slab, const char *state_file);
+/* A slab's sensors and exact clock, which a checkpoint file does not hold
+ * (no counterpart in the reference: its sensors are not checkpointed and its
+ * restart time is the file's %f).  rows: 6 host arrays of the slab's ncell
+ * doubles, input cell order: tact0, tact, trec, Vmax, t90, APD90 phase
+ * (0 waiting, 1 tracking Vmax, 2 done); rows the batch has no sensors for
+ * read -1.  After a set the sensors continue the run (the step's Vm before
+ * is their previous one), so write checkpoint + get sensors, remove, add +
+ * read checkpoint + set sensors resumes an evicted slab exactly. */
+int cdh_batch_get_sensors(cdh_batch *b, int slab, double *rows, double *time, int *loop);
+int cdh_batch_set_sensors(cdh_batch *b, int slab, const double *rows, double time, int loop);
 /* advances every slab by nsteps; asynchronous on the batch's stream */
 int cdh_batch_run(cdh_batch *b, int nsteps);
 int cdh_batch_run_timed(cdh_batch *b, int nsteps, double *ms);

@@ -216,11 +216,14 @@ static void reaction(ora_sim *s)
          gV[i] = s->Vm[ix[i]];
          for (int k = 0; k < ORA_NSTATE; k++) gs[k * m + i] = s->state[k * s->n + ix[i]];
       }
-      /* The reference runs these loops on one thread; per-cell results do not
-       * depend on the split, except the v2 tail rule, which is per group and
-       * is applied below on the whole group. */
+      /* The reference runs these loops on one thread over the whole group.
+       * Per-cell results do not depend on the split except through the v2
+       * loops' per-group rules: the tail rule (cells from 4*floor(m/4) keep
+       * their gates) and the f gate's form (lanes 0..3 of each 12-cell group
+       * below 12*floor(m/12)).  Chunks that start at multiples of 12 see
+       * both rules exactly as the whole group does. */
       int nthr = p->nthreads > 0 ? p->nthreads : 1;
-      int64_t chunk = ((m + nthr - 1) / nthr + 3) / 4 * 4;
+      int64_t chunk = ((m + nthr - 1) / nthr + 11) / 12 * 12;
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
       for (int t = 0; t < nthr; t++) {
          int64_t b = (int64_t)t * chunk;
@@ -232,19 +235,21 @@ static void reaction(ora_sim *s)
             /* per-cell scales: evaluate cell by cell */
             for (int64_t i = b; i < e; i++) {
                sc = &p->scales[ix[i]];
-               if (p->exact) ora_tt06_nongate_exact(p->dt, ct, 1, gV + i, gs + i, m, gd + i);
+               if (p->exact) ora_tt06_nongate_exact(p->dt, ct, sc, 1, gV + i, gs + i, m, gd + i);
                else ora_tt06_nongate(&s->fits, p->dt, ct, sc, 1, gV + i, gs + i, m, gd + i);
             }
          } else {
-            if (p->exact) ora_tt06_nongate_exact(p->dt, ct, cnt, gV + b, gs + b, m, gd + b);
+            if (p->exact) ora_tt06_nongate_exact(p->dt, ct, NULL, cnt, gV + b, gs + b, m, gd + b);
             else ora_tt06_nongate(&s->fits, p->dt, ct, NULL, cnt, gV + b, gs + b, m, gd + b);
          }
          if (p->exact) {
-            ora_tt06_gates_exact(p->dt, ct->s_switch, 0, cnt, gV + b, gs + b, m);
+            ora_tt06_gates_exact(p->dt, ct->s_switch, p->jhTauSmooth, cnt, gV + b, gs + b, m);
          } else {
-            /* chunk boundaries are multiples of 4, so the tail rule of the
-             * whole group lands only in the last chunk */
-            ora_tt06_gates_v2(&s->fits, ct->s_switch, cnt, gV + b, gs + b, m);
+            /* chunk boundaries are multiples of 12: see above */
+            if (p->no_loop_bounds)
+               ora_tt06_gates_v2_all(&s->fits, ct->s_switch, cnt, gV + b, gs + b, m);
+            else
+               ora_tt06_gates_v2(&s->fits, ct->s_switch, cnt, gV + b, gs + b, m);
          }
       }
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
@@ -342,6 +347,25 @@ void ora_sim_set_state(ora_sim *s, const double *in, const double *Vm)
 {
    if (in) memcpy(s->state, in, sizeof(double) * ORA_NSTATE * s->n);
    if (Vm) memcpy(s->Vm, Vm, sizeof(double) * s->n);
+}
+/* A restart as the reference makes one: readCheckpoint sets the states and
+ * Vm (checkpointIO.cc:209-259), the SIMULATE object's loop and time set the
+ * clock, and the sensors, created anew from the deck, start fresh. */
+void ora_sim_restart(ora_sim *s, const double *in, const double *Vm, int loop, double time)
+{
+   ora_sim_set_state(s, in, Vm);
+   s->loop = loop;
+   s->time = time;
+   for (int64_t c = 0; c < s->n; c++) {
+      s->act0[c] = 0;
+      s->tact0[c] = -1.0;
+      s->active[c] = 0;
+      s->tactThr[c] = -1.0;
+      s->trecThr[c] = -1.0;
+      s->apdPhase[c] = 0;
+      s->vmax[c] = 0.0;
+      s->t90[c] = -1.0;
+   }
 }
 void ora_sim_get_activation_vm0(const ora_sim *s, double *out) { memcpy(out, s->tact0, sizeof(double) * s->n); }
 void ora_sim_get_activation_thr(const ora_sim *s, double *out) { memcpy(out, s->tactThr, sizeof(double) * s->n); }

@@ -88,16 +88,23 @@ void ora_tt06_gates_v2(const ora_fitset *fs, int s_switch, int n,
 /* flags of cell i of a reaction group of n cells: bit0 gates fixed (v2 loops
  * stop at 4*floor(n/4)), bit1 f gate in the main-loop lane-0..3 form */
 int  ora_gate_flags(int64_t i, int64_t n);
+/* every cell advanced, the f gate in the tail loop's form (no loop bounds) */
+void ora_tt06_gates_v2_all(const ora_fitset *fs, int s_switch, int n,
+                           const double *Vm, double *state, int64_t stride);
 /* one gate only (eq 0..12 in fit order m,h,j,Xr1,Xr2,Xs,r,d,f,f2,jL,s0,s1) */
 void ora_tt06_gate_v2(const ora_fitset *fs, int eq, int n, const double *Vm, double *g);
 
 /* ---- reaction, configuration A (tolerance 0: exact functions) ---- */
-void ora_tt06_nongate_exact(double dt, const ora_celltype *ct, int n,
-                            const double *Vm, double *state, int64_t stride,
+void ora_tt06_nongate_exact(double dt, const ora_celltype *ct, const ora_scales *sc,
+                            int n, const double *Vm, double *state, int64_t stride,
                             double *dVdt);
 void ora_tt06_gates_exact(double dt, int s_switch, int jhTauSmooth, int n,
                           const double *Vm, double *state, int64_t stride);
-double ora_tt06_exact_func(int idx, double V);   /* idx in fit order, 0..32 */
+/* idx in fit order, 0..32; jhTauSmooth = 1: the smoothed h and j rates */
+double ora_tt06_exact_func(int idx, double V, int jhTauSmooth);
+/* {x0, x1, c0..c3} of the Hermite cubic that smooths the h (which = 0) or
+ * j (which = 1) rate on [x0, x1] (elec/TT06Tau.cc:131-151) */
+void ora_tt06_mod_coefficients(int which, double out[6]);
 
 /* init state for n cells (SoA with stride), dVK_i computed as the reference does */
 void ora_tt06_init_state(const ora_celltype *ct, int n, double *state,
@@ -137,6 +144,8 @@ typedef struct {
    int exact;                   /* 1: configuration A */
    double threshold;            /* ActivationAndRecovery threshold (mV) */
    int nthreads;
+   int jhTauSmooth;             /* configuration A: 1 = smoothed h and j rates (mod = 1) */
+   int no_loop_bounds;          /* 1: every cell's gates advance, f in the tail form */
 } ora_sim_params;
 
 typedef struct {
@@ -151,6 +160,8 @@ void ora_sim_run(ora_sim *s, int nsteps);
 void ora_sim_get_vm(const ora_sim *s, double *out);
 void ora_sim_get_state(const ora_sim *s, double *out /* 19*ncell SoA */);
 void ora_sim_set_state(ora_sim *s, const double *in, const double *Vm);
+/* restart: states and Vm (NULL keeps them), clock, fresh sensors */
+void ora_sim_restart(ora_sim *s, const double *in, const double *Vm, int loop, double time);
 /* first time Vm > 0 (ActivationTimeSensor), -1 if never */
 void ora_sim_get_activation_vm0(const ora_sim *s, double *out);
 /* first up-crossing of the threshold (ActivationAndRecovery), -1 if never */

@@ -11,6 +11,7 @@
  */
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
 #include "pade.hh"
 #include "TT06Func.hh"
@@ -142,6 +143,40 @@ void ref_gate_v2(double dt, int eq, int n, double *Vm, double *g)
 }
 
 double ref_exact_func(int i, double V, int jhTauSmooth) { init_once(); return exact_func(i, jhTauSmooth)(V, NULL); }
+
+/* One row of TT06Func::getStandardCellTypes (TT06Func.cc:421-612):
+ * parms = {P_NaK, g_Ks, g_Kr, g_to, g_NaL, Vm}, state[19] by each STATE's
+ * index (TT06_DEV_STATE_INDEX, TT06Func.h:10-14; K_i itself in slot 1).
+ * Returns the number of states the row lists, or -1 for an unknown name. */
+int ref_celltype_standard(const char *name, int *s_switch, double *parms, double *state)
+{
+   init_once();
+   std::map<std::string, TT06Func::CellTypeParmsFull> t = TT06Func::getStandardCellTypes();
+   auto it = t.find(name);
+   if (it == t.end()) return -1;
+   const TT06Func::CellTypeParmsFull &c = it->second;
+   *s_switch = c.s_switch;
+   parms[0] = c.P_NaK; parms[1] = c.g_Ks; parms[2] = c.g_Kr;
+   parms[3] = c.g_to; parms[4] = c.g_NaL; parms[5] = c.Vm;
+   int cnt = 0;
+   for (const auto &kv : c.state) {
+      if (kv.second.index < 0 || kv.second.index >= 19) return -1;
+      state[kv.second.index] = kv.second.value;
+      cnt++;
+   }
+   return cnt;
+}
+
+/* names getStandardCellTypes defines, space separated, into buf */
+int ref_celltype_names(char *buf, int len)
+{
+   init_once();
+   std::string s;
+   for (const auto &kv : TT06Func::getStandardCellTypes()) s += kv.first + " ";
+   if ((int)s.size() + 1 > len) return -1;
+   std::memcpy(buf, s.c_str(), s.size() + 1);
+   return (int)TT06Func::getStandardCellTypes().size();
+}
 
 void ref_fgr_f2(unsigned iFace, int *tissue, double *gradPhi /* 3*19 */)
 {

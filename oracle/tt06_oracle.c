@@ -333,11 +333,11 @@ void ora_tt06_nongate(const ora_fitset *fs, double dt, const ora_celltype *ct,
    nongate_core(fv05_fit, fv6_fit, fs, dt, ct, sc, n, Vm, state, stride, dVdt);
 }
 
-void ora_tt06_nongate_exact(double dt, const ora_celltype *ct, int n,
-                            const double *Vm, double *state, int64_t stride,
+void ora_tt06_nongate_exact(double dt, const ora_celltype *ct, const ora_scales *sc,
+                            int n, const double *Vm, double *state, int64_t stride,
                             double *dVdt)
 {
-   nongate_core(fv05_exact, fv6_exact, NULL, dt, ct, NULL, n, Vm, state, stride, dVdt);
+   nongate_core(fv05_exact, fv6_exact, NULL, dt, ct, sc, n, Vm, state, stride, dVdt);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -389,12 +389,13 @@ int ora_gate_flags(int64_t i, int64_t n)
    return f;
 }
 
-void ora_tt06_gate_v2(const ora_fitset *fs, int eq, int n, const double *Vm, double *g)
+static void gate_v2_core(const ora_fitset *fs, int eq, int n, const double *Vm, double *g,
+                         int loop_rule)
 {
    ora_tt06_init();
    const ora_pade *pm = &fs->fit[7 + 2 * eq], *pt = &fs->fit[7 + 2 * eq + 1];
    for (int i = 0; i < n; i++) {
-      const int fl = ora_gate_flags(i, n);
+      const int fl = loop_rule ? ora_gate_flags(i, n) : 0;
       if (fl & 1) continue;
       const int form = (eq == 8 && (fl & 2)) ? UPD_SUB : GATE_FORM[eq];
       double mhu = ora_pade_eval(pm, Vm[i]);
@@ -403,12 +404,29 @@ void ora_tt06_gate_v2(const ora_fitset *fs, int eq, int n, const double *Vm, dou
    }
 }
 
+void ora_tt06_gate_v2(const ora_fitset *fs, int eq, int n, const double *Vm, double *g)
+{
+   gate_v2_core(fs, eq, n, Vm, g, 1);
+}
+
 void ora_tt06_gates_v2(const ora_fitset *fs, int s_switch, int n,
                        const double *Vm, double *state, int64_t stride)
 {
    for (int eq = 0; eq < 11; eq++)
       ora_tt06_gate_v2(fs, eq, n, Vm, state + (ORA_M + eq) * stride);
    ora_tt06_gate_v2(fs, s_switch ? 12 : 11, n, Vm, state + ORA_S * stride);
+}
+
+/* The same update forms with every cell of the group advanced and the f gate
+ * in the tail loop's form: the gates of cells whose position in their
+ * reaction group depends on how the anatomy is cut (no counterpart in the
+ * reference, whose groups are per rank). */
+void ora_tt06_gates_v2_all(const ora_fitset *fs, int s_switch, int n,
+                           const double *Vm, double *state, int64_t stride)
+{
+   for (int eq = 0; eq < 11; eq++)
+      gate_v2_core(fs, eq, n, Vm, state + (ORA_M + eq) * stride, 0);
+   gate_v2_core(fs, s_switch ? 12 : 11, n, Vm, state + ORA_S * stride, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -490,6 +508,113 @@ static double sTauR1(double V)
    return 1 / tau;
 }
 
+/* Smoothed h and j rates (mod = 1, jhTauSmooth): TauRecipMod
+ * (elec/TT06Tau.cc:131-151) replaces 1/tau on [x0, x1] by the cubic Hermite
+ * interpolant of 1/tau and its derivative at x0 and x1, the two branches
+ * of hTauRecip / jTauRecip (:14-129).  The derivatives are needed only at
+ * the two ends, where they set the cubic; their operation order follows the
+ * reference, so the cubic's coefficients are the same doubles. */
+static double h_recip(double V, double *dR)
+{
+   double t1, t2, dt1, dt2;
+   if (V < -40.0) {
+      t1 = 0.0570000 * exp(-(V + 80.0) / 6.80000);
+      dt1 = -t1 / 6.8000;
+      t2 = 2.70000 * exp(0.0790000 * V) + 310000. * exp(0.348500 * V);
+      dt2 = 0.079 * 2.70000 * exp(0.0790000 * V) + 0.3485 * 310000. * exp(0.348500 * V);
+   } else {
+      const double c = (0.770000 / 0.130000);
+      const double e = exp((V + 10.6600) / -11.1000);
+      const double s = 1.0 / (1.0 + e);
+      t1 = 0.0;
+      dt1 = 0.0;
+      t2 = c * s;
+      dt2 = c * s * s * e / 11.1;
+   }
+   *dR = dt1 + dt2;
+   return t1 + t2;
+}
+static double j_recip(double V, double *dR)
+{
+   double t1, t2, dt1, dt2;
+   if (V < -40.0) {
+      double a1 = (-25428.0 * exp(0.244400 * V) - 6.94800e-06 * exp(-0.0439100 * V));
+      double da1 = (-0.24400 * 25428.0 * exp(0.244400 * V) + 0.0439100 * 6.94800e-06 * exp(-0.0439100 * V));
+      double a2 = (V + 37.7800);
+      double da2 = 1.0;
+      double c = 0.311;
+      double e = exp(c * (V + 79.2300));
+      double a3 = 1 / (1.0 + e);
+      double da3 = -a3 * a3 * e * c;
+      t1 = a1 * a2 * a3;
+      double q = da1 / a1 + da2 / a2 + da3 / a3;
+      dt1 = q * t1;
+      a1 = 0.0242400 * exp(-0.0105200 * V);
+      da1 = -0.0105200 * a1;
+      c = -0.137800;
+      e = exp(c * (V + 40.1400));
+      a2 = 1.0 / (1.0 + e);
+      da2 = -c * a2 * a2 * e;
+      t2 = a1 * a2;
+      dt2 = (da1 / a1 + da2 / a2) * t2;
+   } else {
+      t1 = 0.0;
+      dt1 = 0.0;
+      const double a1 = 0.6000000 * exp(0.0570000 * V);
+      const double da1 = 0.0570000 * a1;
+      const double c = -0.1;
+      const double e = exp(c * (V + 32.0000));
+      const double a2 = 1.0 / (1.0 + e);
+      const double da2 = -a2 * a2 * e * c;
+      t2 = a1 * a2;
+      dt2 = (da1 / a1 + da2 / a2) * t2;
+   }
+   *dR = dt1 + dt2;
+   return t1 + t2;
+}
+typedef struct { double x0, x1, c[4]; double (*f)(double, double *); } hermite_rate;
+static hermite_rate H_MOD = {-64.20, -23.3, {0, 0, 0, 0}, h_recip};
+static hermite_rate J_MOD = {-48.85, -17.6, {0, 0, 0, 0}, j_recip};
+/* makeTauRecipParms + hermite (TT06Tau.cc:137-157) */
+static void hermite_init(hermite_rate *h)
+{
+   double d0, d1;
+   const double p0 = h->f(h->x0, &d0), p1 = h->f(h->x1, &d1);
+   const double dx = h->x1 - h->x0;
+   h->c[0] = p0;
+   h->c[1] = d0;
+   h->c[2] = 3 * (p1 - p0) / (dx * dx) - (2 * d0 + d1) / dx;
+   h->c[3] = -2 * (p1 - p0) / (dx * dx * dx) + (d0 + d1) / (dx * dx);
+}
+static double hermite_rate_eval(const hermite_rate *h, double V)
+{
+   double d;
+   if (V < h->x0 || V > h->x1) return h->f(V, &d);
+   const double x = V - h->x0;
+   return h->c[0] + x * (h->c[1] + x * (h->c[2] + x * h->c[3]));
+}
+static double hTauRMod(double V) { return hermite_rate_eval(&H_MOD, V); }
+static double jTauRMod(double V) { return hermite_rate_eval(&J_MOD, V); }
+static void mod_init(void)
+{
+   static int done = 0;
+   if (done) return;
+   hermite_init(&H_MOD);
+   hermite_init(&J_MOD);
+   done = 1;
+}
+
+/* the Hermite cubics' coefficients {x0, x1, c0..c3} for h (which = 0) and
+ * j (which = 1) */
+void ora_tt06_mod_coefficients(int which, double out[6])
+{
+   mod_init();
+   const hermite_rate *h = which ? &J_MOD : &H_MOD;
+   out[0] = h->x0;
+   out[1] = h->x1;
+   for (int k = 0; k < 4; k++) out[2 + k] = h->c[k];
+}
+
 typedef double (*gfun)(double);
 static const gfun EXACT_GATE[26] = {
    mMhu, mTauR, hjMhu, hTauR, hjMhu, jTauR, Xr1Mhu, Xr1TauR, Xr2Mhu, Xr2TauR,
@@ -497,9 +622,15 @@ static const gfun EXACT_GATE[26] = {
    jLMhu, jLTauR, sMhu0, sTauR0, sMhu1, sTauR1
 };
 
-double ora_tt06_exact_func(int idx, double V)
+/* exact function idx in fit order (0..32); jhTauSmooth selects the smoothed
+ * h and j rates for idx 10 and 12, as mkFitParms_ names them
+ * (TT06Dev_Reaction.cc:217-219) */
+double ora_tt06_exact_func(int idx, double V, int jhTauSmooth)
 {
    ora_tt06_init();
+   mod_init();
+   if (jhTauSmooth && idx == 10) return hTauRMod(V);
+   if (jhTauSmooth && idx == 12) return jTauRMod(V);
    switch (idx) {
    case 0: return fv0_x(V);
    case 1: return fv1_x(V);
@@ -512,16 +643,20 @@ double ora_tt06_exact_func(int idx, double V)
    }
 }
 
-/* generic updateGate with exact functions (TT06Func.cc:64-161); only the
- * unsmoothed h/j tau forms are restated (the golden uses mod = 0) */
+/* generic updateGate with exact functions (TT06Func.cc:64-161): m by
+ * Rush-Larsen with libm exp, the others g += dt*(mhu - g)*tauR; every cell of
+ * the group (no v2 loop bounds).  jhTauSmooth = 1 takes the smoothed h and j
+ * rates. */
 void ora_tt06_gates_exact(double dt, int s_switch, int jhTauSmooth, int n,
                           const double *VM, double *state, int64_t stride)
 {
    ora_tt06_init();
-   (void)jhTauSmooth;
+   mod_init();
    for (int eq = 0; eq < 12; eq++) {
       int fi = (eq == 11 && s_switch) ? 12 : eq;
       gfun mhuF = EXACT_GATE[2 * fi], tauF = EXACT_GATE[2 * fi + 1];
+      if (jhTauSmooth && eq == 1) tauF = hTauRMod;
+      if (jhTauSmooth && eq == 2) tauF = jTauRMod;
       double *g = state + (ORA_M + eq) * stride;
       for (int ii = 0; ii < n; ii++) {
          double Vm = VM[ii];

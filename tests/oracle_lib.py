@@ -50,7 +50,8 @@ class OraSimParams(C.Structure):
                 ("ncell", C.c_int64), ("gid", i64p), ("celltype", u8p), ("sigma", dp),
                 ("ncelltype", C.c_int), ("celltypes", C.POINTER(OraCelltype)),
                 ("scales", C.POINTER(OraScales)), ("fits", C.POINTER(OraFitset)),
-                ("exact", C.c_int), ("threshold", C.c_double), ("nthreads", C.c_int)]
+                ("exact", C.c_int), ("threshold", C.c_double), ("nthreads", C.c_int),
+                ("jhTauSmooth", C.c_int), ("no_loop_bounds", C.c_int)]
 
 
 class OraBoxStim(C.Structure):
@@ -78,15 +79,16 @@ def oracle():
         L.ora_pade_eval.restype = C.c_double
         L.ora_pade_eval.argtypes = [C.POINTER(OraPade), C.c_double]
         L.ora_tt06_exact_func.restype = C.c_double
-        L.ora_tt06_exact_func.argtypes = [C.c_int, C.c_double]
+        L.ora_tt06_exact_func.argtypes = [C.c_int, C.c_double, C.c_int]
+        L.ora_tt06_mod_coefficients.argtypes = [C.c_int, dp]
         L.ora_celltype_standard.argtypes = [C.c_char_p, C.POINTER(OraCelltype)]
         L.ora_fitset_scale_dt.argtypes = [C.POINTER(OraFitset), C.c_double]
         L.ora_tt06_nongate.argtypes = [C.POINTER(OraFitset), C.c_double, C.POINTER(OraCelltype),
                                        C.POINTER(OraScales), C.c_int, dp, dp, C.c_int64, dp]
         L.ora_tt06_gates_v2.argtypes = [C.POINTER(OraFitset), C.c_int, C.c_int, dp, dp, C.c_int64]
         L.ora_tt06_gate_v2.argtypes = [C.POINTER(OraFitset), C.c_int, C.c_int, dp, dp]
-        L.ora_tt06_nongate_exact.argtypes = [C.c_double, C.POINTER(OraCelltype), C.c_int, dp, dp,
-                                             C.c_int64, dp]
+        L.ora_tt06_nongate_exact.argtypes = [C.c_double, C.POINTER(OraCelltype),
+                                             C.POINTER(OraScales), C.c_int, dp, dp, C.c_int64, dp]
         L.ora_tt06_gates_exact.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int64]
         L.ora_gate_flags.argtypes = [C.c_int64, C.c_int64]
         L.ora_tt06_init_state.argtypes = [C.POINTER(OraCelltype), C.c_int, dp, C.c_int64, dp]
@@ -104,6 +106,7 @@ def oracle():
                    "ora_sim_get_activation_thr", "ora_sim_get_recovery_thr"):
             getattr(L, fn).argtypes = [C.c_void_p, dp]
         L.ora_sim_set_state.argtypes = [C.c_void_p, dp, dp]
+        L.ora_sim_restart.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_double]
         L.ora_sim_get_t90.argtypes = [C.c_void_p, dp]
         L.ora_apd_trace.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     dp, dp]
@@ -135,10 +138,31 @@ def ref():
         L.ref_exact_func.restype = C.c_double
         L.ref_exact_func.argtypes = [C.c_int, C.c_double, C.c_int]
         L.ref_fgr_f2.argtypes = [C.c_uint, ip, dp]
+        L.ref_celltype_standard.argtypes = [C.c_char_p, ip, dp, dp]
+        L.ref_celltype_names.argtypes = [C.c_char_p, C.c_int]
         L.ref_fibre_conductivity.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ref_init()
         _ref = L
     return _ref
+
+
+def ref_celltype_table():
+    """TT06Func::getStandardCellTypes of the reference build: name ->
+    (s_switch, [P_NaK, g_Ks, g_Kr, g_to, g_NaL, Vm], state[19] in slot order
+    with K_i in slot 1)."""
+    R = ref()
+    buf = C.create_string_buffer(4096)
+    n = R.ref_celltype_names(buf, len(buf))
+    names = buf.value.decode().split()
+    assert n == len(names)
+    out = {}
+    for name in names:
+        sw = C.c_int()
+        parms, state = np.zeros(6), np.full(NSTATE, np.nan)
+        cnt = R.ref_celltype_standard(name.encode(), C.byref(sw), ptr(parms), ptr(state))
+        assert cnt == NSTATE, (name, cnt)
+        out[name] = (sw.value, parms, state)
+    return out
 
 
 def aligned(n, dtype=np.float64, align=64):
@@ -184,7 +208,8 @@ class OracleSim:
     """The oracle's omp-loop simulation on an Anatomy (cardioid_amd.workloads)."""
 
     def __init__(self, anat, celltypes, dt, diffusion_scale, stimuli=(), exact=False,
-                 threshold=-40.0, fits=None, scales=None, nthreads=0):
+                 threshold=-40.0, fits=None, scales=None, nthreads=0, jhTauSmooth=0,
+                 gate_tail_rule=True):
         from cardioid_amd.fitfile import read_fit_file
         O = oracle()
         fs = fits or read_fit_file()
@@ -204,7 +229,8 @@ class OracleSim:
                          gid=ptr(self.gid, i64p), celltype=ptr(self.ct, u8p), sigma=ptr(self.sig),
                          ncelltype=len(celltypes), celltypes=self.cts, scales=self.sc,
                          fits=C.pointer(self.ofs), exact=int(exact), threshold=threshold,
-                         nthreads=nthreads)
+                         nthreads=nthreads, jhTauSmooth=int(jhTauSmooth),
+                         no_loop_bounds=int(not gate_tail_rule))
         st = (OraBoxStim * max(1, len(stimuli)))()
         for k, b in enumerate(stimuli):
             for f, _ in OraBoxStim._fields_:
@@ -224,6 +250,13 @@ class OracleSim:
         out = np.zeros(NSTATE * self.n)
         oracle().ora_sim_get_state(self.h, ptr(out))
         return out.reshape(NSTATE, self.n)
+
+    def restart(self, state, vm, loop, time):
+        """States [19][n] and Vm, the clock, and fresh sensors, as the
+        reference restarts from a checkpoint."""
+        st = np.ascontiguousarray(state, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(vm, dtype=np.float64)
+        oracle().ora_sim_restart(self.h, ptr(st), ptr(v), int(loop), float(time))
 
     def activation(self):
         a0, a, r = (np.zeros(self.n) for _ in range(3))

@@ -243,6 +243,20 @@ def test_split_anatomy_with_halo_exchange_equals_whole(mode, parts):
         assert np.array_equal(vm1, vm0)
         assert np.array_equal(st1, st0)
         assert np.array_equal(act1, act0)
+        # the oracle in the loop: the whole anatomy on the CPU, every cell's
+        # gates advanced (a share's reaction groups are not the whole's, so
+        # the v2 loop bounds are off on both sides)
+        import oracle_lib as OL
+        ref = OL.OracleSim(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims,
+                           gate_tail_rule=False)
+        ref.run(450)
+        want_vm, want_act = ref.vm(), ref.activation()[0]
+        ref.close()
+        e = np.abs(vm1 - want_vm) / np.maximum(np.abs(want_vm), 1.0)
+        say(f"  split run vs oracle: Vm max rel {e.max():.3e}, tact0 identical "
+            f"{np.mean(act1 == want_act):.4f}")
+        assert e.max() < 1e-9
+        assert np.array_equal(act1, want_act)
     else:
         assert np.array_equal(act1 >= 0, act0 >= 0)
         assert np.abs(act1 - act0).max() <= DT * 1.0000001
@@ -298,4 +312,65 @@ def test_graph_follows_the_live_tiles(mode):
         vm, n = got[name]
         same = np.array_equal(vm, alone(s, n))
         say(f"  slab {name}: {n} steps, identical to alone: {same}")
+        assert same
+
+
+def test_cfg3_sixty_four_slabs_mixed_autotuned_graphs():
+    """cfg3 at its own configuration: 64 slabs drawn log-uniform from 32^3
+    to 160x160x96 (seed 1; 24.65 M cells), per-slab drug scales on g_Kr,
+    g_CaL and g_Na and fibre angles, box stimulus x < 4, mixed precision,
+    the launch picked by the autotuner, 10-step graphs, 600 steps.  Eight
+    slabs (the smallest, the largest and six between) end bit for bit where
+    they end alone; every slab's activation map is sane: times are -1 or in
+    (0, 12] ms, a slab that fires fires first in its stimulated planes x < 4
+    and its wave moves away from them."""
+    nsteps = 600
+    cases = W.batched_slabs(64, seed=1)
+    slabs = [hip.Slab(c.anatomy, ["endoCellML"], W.DIFFUSION_SCALE, c.stimuli, scales=c.scales)
+             for c in cases]
+    cells, box = capacity(slabs)
+    say(f"cfg3: 64 slabs, {sum(s.ncell for s in slabs)} cells, mixed, autotune, graphs")
+    b = hip.Batch(DT, cells, box, slab_capacity=64, mode="mixed_fused")
+    ids = [b.add(s) for s in slabs]
+    best, table = b.autotune(10)
+    say(f"  autotuned {b.launch()} at {best:.3f} ms/step over {len(table)} candidates")
+    b.set_graph(10)
+    for k in range(3):
+        b.run(nsteps // 3)
+        b.sync()
+        say(f"  {(k + 1) * nsteps // 3} steps")
+    order = np.argsort([s.ncell for s in slabs])
+    check = sorted(set(order[np.linspace(0, 63, 8).round().astype(int)].tolist()))
+    got = {k: (b.vm(ids[k]), b.activation(ids[k])[1]) for k in check}
+    acts = [b.activation(ids[k])[1] for k in range(64)]
+    b.close()
+    end = nsteps * DT
+    fired = 0
+    for k, (c, act) in enumerate(zip(cases, acts)):
+        a = c.anatomy
+        x = a.gid % a.nx
+        on = act >= 0
+        assert np.all((act == -1.0) | ((act > 0) & (act <= end + 1e-9))), k
+        if not on.any():
+            continue          # drug block (s_Na, s_CaL near 0) can stop a slab firing
+        fired += 1
+        # the first cells to activate lie in the stimulated planes x < 4, and
+        # the wave moves away from them: the mean activation time of the
+        # fully activated x-planes does not decrease with x
+        assert act[on].min() == act[on & (x < 4)].min(), k
+        planes = [act[x == i].mean() for i in range(a.nx) if on[x == i].all()]
+        assert np.all(np.diff(planes) >= -DT), k
+    say(f"  {fired} of 64 slabs activated")
+    assert fired >= 32
+    for k in check:
+        s = slabs[k]
+        c1, x1 = capacity([s])
+        one = hip.Batch(DT, c1, x1, slab_capacity=1, mode="mixed_fused")
+        sid = one.add(s)
+        one.run(nsteps)
+        vm, act = one.vm(sid), one.activation(sid)[1]
+        one.close()
+        same = np.array_equal(vm, got[k][0]) and np.array_equal(act, got[k][1])
+        say(f"  slab {k}: {cases[k].anatomy.nx}x{cases[k].anatomy.ny}x{cases[k].anatomy.nz}, "
+            f"activated {(act >= 0).mean():.3f}, alone == in the batch: {same}")
         assert same

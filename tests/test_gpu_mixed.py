@@ -1,9 +1,12 @@
-"""Mixed-precision fused engine against the FP64 engine on the same inputs.
+"""Mixed-precision fused engine against the FP64 oracle on the same inputs.
 
-The FP64 engine is the oracle-pinned drop-in (test_gpu_parity); the mixed
-form keeps Vm, Ca_i and Na_i in FP64 and the other states and the stencil in
-FP32.  Bar: activation times within one time step, APD90 within 1 %, and
-the L-infinity Vm error after the action potential reported and bounded.
+The mixed form keeps Vm, Ca_i and Na_i in FP64 and the other states and
+the stencil in FP32.  Bar, against the oracle's FP64 run over 500 ms:
+activation times within one time step, APD90 within 1 %, the same cells
+activated and repolarised, and the L-infinity Vm error reported and
+bounded.  On the full 256x256x512 anatomy, where the oracle takes hours,
+the mixed run is held to the FP64 engine (itself within 1e-14 of the
+oracle) over 5000 steps.
 """
 import sys
 
@@ -38,28 +41,82 @@ def run(a, stims, mode, n, scales=None):
     return out
 
 
-@pytest.mark.parametrize("nsteps", [25000])
-def test_mixed_against_fp64_on_shell(nsteps):
-    say(f"mixed vs fp64: shell 32x28x40, 8 endocardial stimuli, {nsteps} steps")
-    a, depth = W.ellipsoid_shell(32, 28, 40, seed=0)
-    stims = W.endocardial_point_stimuli(a, depth, 8, seed=0)
-    ref = run(a, stims, "fp64_fused", nsteps)
-    mix = run(a, stims, "mixed_fused", nsteps)
+def errors_vs(ref, mix):
+    """(activation |dt| max, fraction identical, APD90 rel max, rel mean,
+    repolarised fraction, L-inf Vm)"""
     dact = np.abs(mix["act"] - ref["act"])
     ok = (ref["t90"] > 0) & (mix["t90"] > 0)
     apd_ref = ref["t90"] - ref["act"]
     apd_mix = mix["t90"] - mix["act"]
-    rel = np.abs(apd_mix - apd_ref)[ok] / apd_ref[ok]
-    linf = np.abs(mix["vm"] - ref["vm"]).max()
-    say(f"  activation |dt| max {dact.max():.4f} ms (dt {DT}), identical {np.mean(dact == 0):.4f}")
-    say(f"  APD90 rel err max {rel.max():.2e} mean {rel.mean():.2e}, "
-        f"repolarised {ok.mean():.3f}")
+    rel = np.abs(apd_mix - apd_ref)[ok] / apd_ref[ok] if ok.any() else np.zeros(1)
+    return (dact.max(), np.mean(dact == 0), rel.max(), rel.mean(), ok.mean(),
+            np.abs(mix["vm"] - ref["vm"]).max())
+
+
+def test_mixed_against_the_fp64_oracle_over_500_ms():
+    """cfg2's measurement on the oracle's scale: the 32x28x40 ellipsoidal
+    shell (endo/mid/epi, rotating fibres, 8 endocardial stimuli), 25000
+    steps of 0.02 ms, mixed engine against the oracle's FP64 run."""
+    import oracle_lib as OL
+    nsteps = 25000
+    say(f"mixed vs FP64 oracle: shell 32x28x40, 8 endocardial stimuli, {nsteps} steps")
+    a, depth = W.ellipsoid_shell(32, 28, 40, seed=0)
+    stims = W.endocardial_point_stimuli(a, depth, 8, seed=0)
+    mix = run(a, stims, "mixed_fused", nsteps)
+    ora = OL.OracleSim(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims)
+    for k in range(5):
+        ora.run(nsteps // 5)
+        say(f"  oracle {(k + 1) * nsteps // 5} steps")
+    ref = dict(vm=ora.vm(), act=ora.activation()[1], t90=ora.t90())
+    ora.close()
+    dact, same, apd_max, apd_mean, repol, linf = errors_vs(ref, mix)
+    say(f"  activation |dt| max {dact:.4f} ms (dt {DT}), identical {same:.4f}")
+    say(f"  APD90 rel err max {apd_max:.2e} mean {apd_mean:.2e}, repolarised {repol:.3f}")
     say(f"  L-inf Vm after {nsteps * DT:.0f} ms: {linf:.3e} mV")
     assert (ref["act"] >= 0).all() and (mix["act"] >= 0).all()
-    assert dact.max() <= DT * 1.0000001
-    assert ok.mean() > 0.99 and np.array_equal(ref["t90"] > 0, mix["t90"] > 0)
-    assert rel.max() < 0.01
+    assert dact <= DT * 1.0000001
+    assert repol > 0.99 and np.array_equal(ref["t90"] > 0, mix["t90"] > 0)
+    assert apd_max < 0.01
     assert linf < 1.0
+
+
+def test_mixed_on_the_full_anatomy_against_fp64():
+    """cfg2's anatomy at its own size: 256x256x512 ellipsoidal shell (20.1 M
+    tissue cells), 5000 steps (100 ms) of the mixed engine against the FP64
+    engine with fused multiply-adds in the fits (within 1e-14 of the oracle):
+    the same cells activated, activation times within one step, Vm within a
+    bound while the tissue is in its plateau."""
+    nsteps = 5000
+    say("mixed vs fp64_fma on 256x256x512 (20.1 M cells), 5000 steps")
+    a, depth = W.ellipsoid_shell(256, 256, 512, seed=0)
+    stims = W.endocardial_point_stimuli(a, depth, 8, seed=0)
+    out = {}
+    for mode in ("fp64_fma", "mixed_fused"):
+        sim = hip.Simulation(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims, mode=mode)
+        for k in range(5):
+            sim.run(nsteps // 5)
+            sim.sync()
+            say(f"  {mode} {(k + 1) * nsteps // 5} steps")
+        out[mode] = dict(vm=sim.vm(), act=sim.activation()[1])
+        sim.close()
+    ref, mix = out["fp64_fma"], out["mixed_fused"]
+    end = nsteps * DT
+    on, on_mix = ref["act"] >= 0, mix["act"] >= 0
+    both = on & on_mix
+    dact = np.abs(mix["act"] - ref["act"])[both]
+    # a cell activated in one run only was reached in the run's last step
+    only = on ^ on_mix
+    last = np.where(on, ref["act"], mix["act"])[only]
+    dv = np.abs(mix["vm"] - ref["vm"])
+    plateau = both & (ref["act"] < end - 10.0)
+    say(f"  activated {on.mean():.4f}; activation |dt| max {dact.max():.4f} ms, identical "
+        f"{np.mean(dact == 0):.4f}; in one run only {only.sum()}")
+    say(f"  L-inf Vm {dv.max():.3e} mV; over the {plateau.mean():.3f} activated more than "
+        f"10 ms before the end {dv[plateau].max():.3e} mV")
+    assert on.mean() > 0.3
+    assert dact.max() <= DT * 1.0000001
+    assert (last >= end - DT * 1.0000001).all()
+    assert dv[plateau].max() < 1.0
 
 
 def test_mixed_state_roundtrip_and_drug_scales():

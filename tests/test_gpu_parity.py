@@ -99,6 +99,64 @@ def test_reaction_plugin_matches_oracle(name, scales):
     assert np.mean(got_st == st) > 0.99
 
 
+@pytest.mark.parametrize("mod", [0, 1])
+@pytest.mark.parametrize("name", ["endoCellML", "midCellML", "epiCellML", "midRRG"])
+def test_reaction_configuration_A_matches_oracle(name, mod):
+    """The factory's default deck (tolerance 0, mod 0 or 1, fastGate 0,
+    fastNonGate 0; no fit file): the exact TT06 functions with the generic
+    gate update, one calc against the oracle's configuration A (pinned
+    bitwise against the reference's own TT06Func / TT06NonGates / TT06Tau
+    kernels).  The device exp / log / sqrt / pow may round differently
+    from glibc's, so the bar is 1e-12 relative per state and for dVm."""
+    import oracle_lib as OL
+    say(f"reaction configuration A {name} mod={mod}")
+    L = hip.lib()
+    n = 1003
+    rng = np.random.default_rng(17)
+    ct = OL.celltype(name)
+    st = np.zeros(19 * n)
+    vm = np.zeros(n)
+    OL.oracle().ora_tt06_init_state(C.byref(ct), n, OL.ptr(st), n, OL.ptr(vm))
+    S = st.reshape(19, n)
+    # voltages over the whole physiological range and both sides of the
+    # smoothed h / j intervals and the -40 mV branch point
+    vm[:] = rng.uniform(-95, 45, n)
+    vm[:8] = [-64.2, -23.3, -48.85, -17.6, -40.0, np.nextafter(-40.0, 0), -91.0, 15.0]
+    S[:] *= rng.uniform(0.8, 1.2, (19, n))
+    S[1] = 138.0 * rng.uniform(0.95, 1.05, n) / OL.oracle().ora_c9() + vm
+    init = st.copy()
+    p = hip.TT06Params(dt=DT, tolerance=0.0, mod=mod, fastGate=0, fastNonGate=0,
+                       fit_l=None, fit_m=None, fit_coef=None,
+                       celltype=hip.standard_celltype(name), s_kr=1.0, s_cal=1.0, s_na=1.0)
+    h = C.c_void_p()
+    hip.check(L.cdh_tt06_create(C.byref(p), n, C.byref(h)), "create")
+    try:
+        hip.check(L.cdh_tt06_set_state(h, hip.ptr(st)), "set_state")
+        perm = rng.permutation(n).astype(np.int32)
+        Vfull = np.zeros(n)
+        Vfull[perm] = vm
+        d_idx, d_vm, d_dvm = dev(perm), dev(Vfull), dev(np.full(n, np.nan))
+        hip.check(L.cdh_tt06_calc(h, DT, dp(d_idx), n, dp(d_vm), None, dp(d_dvm), None), "calc")
+        torch.cuda.synchronize()
+        got_st = np.zeros(19 * n)
+        hip.check(L.cdh_tt06_get_state(h, hip.ptr(got_st)), "get_state")
+        got_dvm = d_dvm.cpu().numpy()[perm]
+    finally:
+        L.cdh_tt06_destroy(h)
+    dv = np.zeros(n)
+    OL.oracle().ora_tt06_nongate_exact(DT, C.byref(ct), None, n, OL.ptr(vm), OL.ptr(st), n,
+                                       OL.ptr(dv))
+    OL.oracle().ora_tt06_gates_exact(DT, ct.s_switch, mod, n, OL.ptr(vm), OL.ptr(st), n)
+    e_dv = np.abs(got_dvm - dv) / np.maximum(np.abs(dv), 1.0)
+    e_st = np.abs(got_st - st) / np.maximum(np.abs(st), 1e-300)
+    say(f"  bitwise dVm {np.mean(got_dvm == dv):.4f} state {np.mean(got_st == st):.4f} "
+        f"max rel {e_dv.max():.2e} {e_st.max():.2e}")
+    assert e_dv.max() < 1e-12 and e_st.max() < 1e-12
+    # the generic loop has no v2 loop bounds: the last n % 4 cells' gates
+    # move too (m, the Rush-Larsen gate, in every cell)
+    assert (got_st.reshape(19, n)[7] != init.reshape(19, n)[7]).all()
+
+
 def test_fgr_plugin_matches_oracle_with_remote_cells():
     import oracle_lib as OL
     from test_oracle_fgr import apply, build, spd_tensor
@@ -192,6 +250,40 @@ def test_cfg0_full_size_matches_oracle():
     say("engine cfg0: brick 64^3 endo, box stimulus x < 4, 1000 steps")
     a = W.brick(64, 64, 64)
     _engine_vs_oracle(a, ["endoCellML"], [W.box_stimulus(xMax=4, nx=64, ny=64, nz=64)], 1000)
+
+
+@pytest.mark.parametrize("mod,size", [(0, 64), (1, 24)])
+def test_cfg0_configuration_A_matches_oracle(mod, size):
+    """cfg0 with the factory's default reaction deck (tolerance 0: exact
+    functions, generic gate update), in the reference's loop over the drop-in
+    objects: 64^3 endocardial brick for mod 0 (the default), a 24^3 brick for
+    mod 1; 1000 steps.  Vm within 1e-9 relative and activation times
+    identical to the oracle's configuration A."""
+    import oracle_lib as OL
+    say(f"cfg0 configuration A, brick {size}^3, mod={mod}, 1000 steps")
+    a = W.brick(size, size, size)
+    stims = [W.box_stimulus(xMax=4, nx=size, ny=size, nz=size)]
+    ref = OL.OracleSim(a, ["endoCellML"], DT, W.DIFFUSION_SCALE, stims, exact=True,
+                       jhTauSmooth=mod)
+    for k in range(4):
+        ref.run(250)
+        say(f"  oracle {250 * (k + 1)} steps")
+    sim = hip.Simulation(a, ["endoCellML"], DT, W.DIFFUSION_SCALE, stims, mode="fp64_unfused",
+                         exact_functions=True, jhTauSmooth=mod)
+    sim.run(1000)
+    vm, act = sim.vm(), sim.activation()
+    want_vm, want_act = ref.vm(), ref.activation()
+    assert abs(sim.time - ref.time) == 0.0
+    e = relerr(vm, want_vm)
+    say(f"  Vm max rel {e.max():.3e} bitwise {np.mean(vm == want_vm):.4f}")
+    assert e.max() < 1e-9
+    for k, nm in enumerate(("tact0", "tact", "trec")):
+        same = act[k] == want_act[k]
+        say(f"  {nm} identical {same.mean():.4f}, activated {np.mean(want_act[k] >= 0):.3f}")
+        assert same.all(), nm
+    assert (want_act[1] >= 0).mean() > 0.5
+    sim.close()
+    ref.close()
 
 
 def test_engine_shell_three_celltypes_matches_oracle():

@@ -16,7 +16,8 @@ import pytest
 
 from cardioid_amd.fitfile import FIT_ORDER, read_fit_file
 from oracle_lib import (OraBoxStim, OraCelltype, OraSimParams, aligned, celltype, i64p,
-                        make_fitset, oracle, ptr, ref_available, ref_load, u8p)
+                        make_fitset, oracle, ptr, ref, ref_available, ref_celltype_table,
+                        ref_load, u8p)
 
 needs_ref = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built (reference tree absent)")
 FS = read_fit_file()
@@ -45,14 +46,13 @@ def test_fits_approximate_exact_functions():
     ofs = make_fitset(FS)
     for i, f in enumerate(FS.fits):
         V = np.arange(f.x0, f.x1 + 1e-9, f.deltaX)
-        ex = np.array([O.ora_tt06_exact_func(i, v) for v in V])
+        ex = np.array([O.ora_tt06_exact_func(i, v, 1) for v in V])
         fit = np.array([O.ora_pade_eval(C.byref(ofs.fit[i]), v) for v in V])
         rng = ex.max() - ex.min()
         err = np.abs(fit - ex).max()
-        # the h/j tau fits approximate the Hermite-smoothed functions (mod=1),
-        # which differ from the raw closed forms near the bridged intervals
-        bound = 5e-2 if f.name in ("hTauRMod", "jTauRMod") else 5 * f.tol
-        assert err <= bound * rng, (f.name, err / rng)
+        # the fit file was made with mod = 1: its h/j tau fits approximate the
+        # Hermite-smoothed rates, which the oracle evaluates at jhTauSmooth = 1
+        assert err <= 5 * f.tol * rng, (f.name, err / rng)
 
 
 def test_singlecell_golden_configuration_A():
@@ -102,11 +102,16 @@ def _random_cells(ct, n, P, rng):
 
 
 @needs_ref
-@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("config", ["B", "A", "A-mod"])
 @pytest.mark.parametrize("name", ["endoCellML", "midCellML", "epiCellML", "midRRG"])
-def test_reaction_bitwise_against_reference_kernels(exact, name):
+def test_reaction_bitwise_against_reference_kernels(config, name):
+    """Configuration B (fits, v2 gates) and configuration A (exact
+    functions, generic gates) with mod = 0 and mod = 1 (smoothed h and j
+    rates), each against the reference's own kernels."""
+    exact = config != "B"
+    mod = 0 if config == "A" else 1
     O = oracle()
-    R = ref_load(FS, DT, exact=exact, jhTauSmooth=0 if exact else 1)
+    R = ref_load(FS, DT, exact=exact, jhTauSmooth=mod)
     ofs = make_fitset(FS, DT)
     ct = celltype(name)
     R.ref_set_celltype(ct.s_switch, ct.P_NaK, ct.g_Ks, ct.g_Kr, ct.g_to, ct.g_NaL)
@@ -118,18 +123,82 @@ def test_reaction_bitwise_against_reference_kernels(exact, name):
     d1, d2 = aligned(n), aligned(n)
     R.ref_nongate(DT, n, P, ptr(Vm), ptr(st), ptr(d1))
     if exact:
-        O.ora_tt06_nongate_exact(DT, C.byref(ct), n, ptr(Vm), ptr(st2), P, ptr(d2))
+        O.ora_tt06_nongate_exact(DT, C.byref(ct), None, n, ptr(Vm), ptr(st2), P, ptr(d2))
     else:
         O.ora_tt06_nongate(C.byref(ofs), DT, C.byref(ct), None, n, ptr(Vm), ptr(st2), P, ptr(d2))
     assert np.array_equal(d1, d2)
     assert np.array_equal(st, st2)
     if exact:
         R.ref_gates_generic(DT, n, P, ptr(Vm), ptr(st))
-        O.ora_tt06_gates_exact(DT, ct.s_switch, 0, n, ptr(Vm), ptr(st2), P)
+        O.ora_tt06_gates_exact(DT, ct.s_switch, mod, n, ptr(Vm), ptr(st2), P)
     else:
         R.ref_gates_fast(DT, n, P, ptr(Vm), ptr(st))
         O.ora_tt06_gates_v2(C.byref(ofs), ct.s_switch, n, ptr(Vm), ptr(st2), P)
     assert np.array_equal(st, st2)
+
+
+@needs_ref
+@pytest.mark.parametrize("mod", [0, 1])
+def test_exact_functions_bitwise_against_reference(mod):
+    """All 33 exact functions of configuration A against the reference's
+    (TT06Func.cc:261-417, TT06NonGates.c:319-364, TT06Tau.cc), over
+    V in [-110, 140] on a 0.01 mV grid plus the ends of the smoothed h and
+    j intervals and their neighbouring doubles."""
+    O, R = oracle(), ref()
+    V = list(np.arange(-110.0, 140.0, 0.01))
+    for x in (-64.20, -23.3, -48.85, -17.6, -40.0):
+        V += [np.nextafter(x, -np.inf), x, np.nextafter(x, np.inf)]
+    for i in range(33):
+        a = np.array([O.ora_tt06_exact_func(i, v, mod) for v in V])
+        b = np.array([R.ref_exact_func(i, v, mod) for v in V])
+        assert np.array_equal(a, b), (i, mod)
+    # the smoothing cubic meets the raw rate at both ends of its interval
+    c = np.zeros(6)
+    for which, idx in ((0, 10), (1, 12)):
+        O.ora_tt06_mod_coefficients(which, ptr(c))
+        x0, x1 = c[0], c[1]
+        for x in (x0, x1):
+            raw = O.ora_tt06_exact_func(idx, np.nextafter(x, -np.inf if x == x0 else np.inf), 0)
+            assert abs(O.ora_tt06_exact_func(idx, x, 1) - raw) <= 1e-9 * abs(raw)
+
+
+@needs_ref
+def test_celltype_tables_equal_the_reference():
+    """Every row of TT06Func::getStandardCellTypes (TT06Func.cc:421-612), as
+    the reference build returns it, equals the oracle's table and the HIP
+    library's (cdh_celltype_standard) bit for bit: s_switch, the five
+    conductances, the rest Vm and the 19 rest states."""
+    from cardioid_amd import hip
+    table = ref_celltype_table()
+    assert set(table) == {"endoCellML", "midCellML", "epiCellML", "endoRRG", "midRRG", "epiRRG"}
+    for name, (sw, parms, state) in table.items():
+        for ct in (celltype(name), hip.standard_celltype(name)):
+            got = np.array([ct.P_NaK, ct.g_Ks, ct.g_Kr, ct.g_to, ct.g_NaL, ct.Vm0])
+            assert ct.s_switch == sw, name
+            assert np.array_equal(got, parms), (name, got, parms)
+            assert np.array_equal(np.array(ct.state0[:]), state), name
+
+
+def test_threaded_oracle_equals_its_serial_run():
+    """The reference advances each reaction group on one thread; the oracle
+    splits a group across threads in chunks that start at multiples of 12,
+    where the v2 loops' per-group rules (tail bound, f-gate form) fall as in
+    the whole group.  1, 5 and 8 threads give the same doubles, with and
+    without the loop bounds."""
+    from cardioid_amd import workloads as W
+    from oracle_lib import OracleSim
+    a, depth = W.ellipsoid_shell(24, 22, 30, seed=3)
+    stims = W.endocardial_point_stimuli(a, depth, 3, seed=3)
+    for rule in (True, False):
+        out = []
+        for nt in (1, 5, 8):
+            r = OracleSim(a, ["endoCellML", "midCellML", "epiCellML"], DT, 714.2857143, stims,
+                          nthreads=nt, gate_tail_rule=rule)
+            r.run(120)
+            out.append((r.vm(), r.state()))
+            r.close()
+        for vm, st in out[1:]:
+            assert np.array_equal(vm, out[0][0]) and np.array_equal(st, out[0][1])
 
 
 @needs_ref
