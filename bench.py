@@ -51,40 +51,86 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(budget_s: float = 12.0):
-    """The oracle (a C/OpenMP restatement of the reference CPU path) on a
-    bounded sample of the same workload: a 96x96x192 ellipsoidal shell from
-    the same generator, as many steps as fit in `budget_s`."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as OL
-    from cardioid_amd import workloads as W
-    threads = min(16, os.cpu_count() or 1)
-    a, depth = W.ellipsoid_shell(96, 96, 192, seed=0)
-    stims = W.endocardial_point_stimuli(a, depth, 8, seed=0)
-    sim = OL.OracleSim(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims, nthreads=threads)
-    sim.run(2)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        sim.run(5)
-        steps += 5
-        el = time.perf_counter() - t0
-        if el > budget_s:
-            break
-    sim.close()
-    rate = a.ncell * steps / el / 1e9
-    cpu = "unknown"
+def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": rate, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"96x96x192 ellipsoidal shell ({a.ncell} tissue cells), {steps} steps, "
-                      f"{el:.1f} s, oracle/liboracle.so with {threads} OpenMP threads on {cpu}"}
+    return "unknown"
+
+
+def host_threads():
+    """The host threads this run may use: OMP_NUM_THREADS where the host
+    sets it (a GPU box's share of its CPUs), else every CPU."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return n if n > 0 else (os.cpu_count() or 1)
+
+
+def _time_oracle(sim, budget_s, chunk=5):
+    sim.run(2)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        sim.run(chunk)
+        steps += chunk
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            return steps, el
+
+
+def cpu_baseline(budget_s: float = 12.0):
+    """The CPU path on a bounded sample of the same workload (a 96x96x192
+    ellipsoidal shell from the same generator, 1.06 M tissue cells, as many
+    steps as fit in `budget_s`), extrapolated per cell-step to the 20.1 M-cell
+    shell.  Two forms:
+      - port: the oracle (the C/OpenMP restatement of the reference CPU
+        path) with every loop threaded;
+      - reference: the same loop with the reaction run by the reference's own
+        kernels (TT06NonGates.c update_nonGate and the TT06GatesSimd.c v2
+        gates, built from source into oracle/_ref), (i) as the reference runs
+        it, one thread per reaction group and serial host loops, and (ii)
+        threaded over cells; stencil and loop are the restatement's, since
+        FGRDiffusionOMP.cc and simulationLoop.cc need <mpi.h>, which the image
+        lacks.  Present where oracle/_ref was built."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    from cardioid_amd import workloads as W
+    threads = host_threads()
+    a, depth = W.ellipsoid_shell(96, 96, 192, seed=0)
+    stims = W.endocardial_point_stimuli(a, depth, 8, seed=0)
+    host = (f"{cpu_model()}, nproc {os.cpu_count()}, {threads} OpenMP threads used "
+            f"(OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS', 'unset')})")
+    sim = OL.OracleSim(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims, nthreads=threads)
+    steps, el = _time_oracle(sim, budget_s)
+    sim.close()
+    out = {"value": a.ncell * steps / el / 1e9, "unit": "Gcell-steps/s", "cores": threads,
+           "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "sample": f"96x96x192 ellipsoidal shell ({a.ncell} tissue cells, the 256x256x512 "
+                     f"workload's generator), {steps} steps in {el:.1f} s, oracle/liboracle.so; "
+                     f"rate per tissue cell-step extrapolated to the 20.1 M-cell shell; {host}"}
+    if OL.ref_available():
+        ref = {}
+        for form, (rt, pt) in (("i_as_is", (1, 1)), ("ii_threaded", (threads, threads))):
+            sim = OL.OracleSim(a, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims,
+                               nthreads=threads)
+            sim.use_reference_reaction(rt, pt)
+            steps, el = _time_oracle(sim, budget_s / 2, chunk=1 if rt == 1 else 5)
+            sim.close()
+            ref[form] = {"value": a.ncell * steps / el / 1e9, "steps": steps, "seconds": el,
+                         "reaction_threads": rt}
+        ref.update(unit="Gcell-steps/s", kind="reference", cores=threads,
+                   sample="the same shell and loop, the reaction by the reference's own "
+                          "update_nonGate and v2 gate kernels (oracle/_ref); (i) one thread per "
+                          "reaction group and serial integrator and sensors, as the reference "
+                          "runs them; (ii) every loop threaded")
+        out["reference_kernels"] = ref
+    return out
 
 
 def load_profile(stem):
@@ -131,24 +177,75 @@ def stream_ceiling(hip, cells, reps=20):
     return out
 
 
-def fused_run(hip, slab, ncell, mode, steps, warmup, tune_steps, graph):
-    """One slab in the engine: autotuned launch, graph replay, HIP-event time."""
+def fused_run(hip, slab, ncell, mode, steps, warmup, tune_steps, graph, sensors=1,
+              prebuild=True):
+    """One slab in the engine: autotuned launch, graph replay, HIP-event time.
+    The set-up (autotune, then one graph length of steps that captures the
+    graph, plus one step when the warm-up count is odd) runs before the
+    warm-up, so that the timed steps start on a graph boundary with the graph
+    built: the timed region replays graphs and builds none."""
     cells, box, _ = slab.extent()
-    b = hip.Batch(DT, cells, box, slab_capacity=1, mode=mode)
+    b = hip.Batch(DT, cells, box, slab_capacity=1, mode=mode, sensors=sensors)
     sid = b.add(slab)
     b.sync()
     best, table = b.autotune(tune_steps) if tune_steps > 0 else (None, [])
     b.set_graph(graph)
+    setup = (graph + (warmup % 2)) if graph > 0 and prebuild else 0
+    b.run(setup)
     b.run(warmup)
     b.sync()
     ms = b.run_timed(steps)
     vm = b.vm(sid)
     act = b.activation(sid)[1]
+    t90 = b.t90(sid) if sensors & 2 else None
     rng = b.range_report(sid)
     launch = b.launch()
     b.close()
     return dict(ms=ms, steps=steps, rate=ncell * steps / (ms * 1e-3) / 1e9, vm=vm, act=act,
-                launch=launch, tune=table, range=rng)
+                t90=t90, launch=launch, tune=table, range=rng, setup_steps=setup)
+
+
+def launch_json(launch):
+    return {"waves": launch[0], "block": launch[1], "occupancy": launch[2],
+            "tile_order": launch[3]}
+
+
+def tune_json(table):
+    """the autotuner's table: (workgroup size, occupancy cap, tile order, ms
+    per step) per candidate"""
+    return [[r[1], r[2], r[3], round(r[4], 5)] for r in table]
+
+
+def accuracy(np, hip, slab, ncell, steps, graph):
+    """Mixed against FP64 on the same anatomy, the same number of steps:
+    L-infinity Vm at the end, activation-time error, APD90 error.  The FP64
+    run is the fused engine with fused multiply-adds in the fits, within
+    1e-14 of the oracle (tests/test_gpu_parity.py); the oracle itself would
+    take hours at 20.1 M cells."""
+    runs = {}
+    for mode in ("fp64_fma", "mixed_fused"):
+        t = time.perf_counter()
+        runs[mode] = fused_run(hip, slab, ncell, mode, steps, 0, 0, graph, sensors=3,
+                               prebuild=False)
+        log(f"accuracy: {mode} {steps} steps ({time.perf_counter() - t:.1f} s)")
+    f, m = runs["fp64_fma"], runs["mixed_fused"]
+    on_f, on_m = f["act"] >= 0, m["act"] >= 0
+    both = on_f & on_m
+    dact = np.abs(m["act"] - f["act"])[both]
+    apd_f, apd_m = f["t90"] - f["act"], m["t90"] - m["act"]
+    rep = both & (f["t90"] > 0) & (m["t90"] > 0)
+    rel = np.abs(apd_m - apd_f)[rep] / apd_f[rep]
+    return {"reference": "fp64_fma (fused FP64, within 1e-14 of the oracle)", "steps": steps,
+            "t_ms": steps * DT,
+            "linf_vm_mV": float(np.abs(m["vm"] - f["vm"]).max()),
+            "activation_max_abs_ms": float(dact.max()) if dact.size else None,
+            "activation_identical_frac": float(np.mean(dact == 0)) if dact.size else None,
+            "activated_fp64": int(on_f.sum()),
+            "activated_in_one_run_only": int((on_f != on_m).sum()),
+            "apd90_rel_max": float(rel.max()) if rel.size else None,
+            "apd90_rel_mean": float(rel.mean()) if rel.size else None,
+            "repolarised_fp64": int((f["t90"] > 0).sum()),
+            "repolarised_in_one_run_only": int(((f["t90"] > 0) != (m["t90"] > 0)).sum())}
 
 
 def batched(hip, W, steps, tune_steps, graph):
@@ -165,7 +262,7 @@ def batched(hip, W, steps, tune_steps, graph):
     log(f"batched: {len(slabs)} slabs, {ncell} cells, added in {time.perf_counter() - t:.1f} s")
     best, table = b.autotune(tune_steps)
     b.set_graph(graph)
-    b.run(10)
+    b.run(graph)
     b.sync()
     ms = b.run_timed(steps)
     act = [b.activation(i)[1] for i in ids]
@@ -174,7 +271,7 @@ def batched(hip, W, steps, tune_steps, graph):
     return {"value": ncell * steps / (ms * 1e-3) / 1e9, "unit": "Gcell-steps/s",
             "slabs": len(slabs), "tissue_cells": ncell, "steps": steps,
             "ms_per_step": ms / steps,
-            "launch": {"waves": launch[0], "block": launch[1], "occupancy": launch[2]},
+            "launch": launch_json(launch), "autotune": tune_json(table),
             "graph_steps": graph,
             "slabs_activated": int(sum((a >= 0).mean() > 0.5 for a in act)),
             "achieved_GBps": MIXED_BYTES * ncell * steps / (ms * 1e-3) / 1e9,
@@ -227,7 +324,7 @@ def serving(hip, W, nreq, rate, hbm_gb, graph, max_active_cells=None):
     reqs = W.request_stream(nreq, rate_hz=rate, seed=4)
     cells, box = serve.arena_for_budget(int(hbm_gb * 2 ** 30))
     b = hip.Batch(DT, cells, box, slab_capacity=1024, mode="mixed_fused", sensors=3)
-    b.set_launch(3, 64)
+    b.set_launch(3, 64, 0, -1)
     b.set_graph(graph)
     st = serve.Server(b, chunk=100, max_active_cells=max_active_cells).serve(reqs)
     b.close()
@@ -250,11 +347,17 @@ def main():
     ap.add_argument("--exact-steps", type=int, default=2000)
     ap.add_argument("--tune-steps", type=int, default=40)
     ap.add_argument("--graph", type=int, default=10)
-    ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--batched-steps", type=int, default=10000)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
-    ap.add_argument("--serve-cap", type=int, default=6000000)
+    ap.add_argument("--serve-cap", type=int, default=0,
+                    help="also serve the stream with at most this many cells in flight")
+    ap.add_argument("--steady-requests", type=int, default=100,
+                    help="a second stream below capacity: this many requests ...")
+    ap.add_argument("--steady-rate", type=float, default=1.5, help="... at this rate (req/s)")
+    ap.add_argument("--accuracy-steps", type=int, default=25000,
+                    help="mixed against FP64 on the headline anatomy over this many steps")
     ap.add_argument("--shard-parts", type=int, default=8)
     ap.add_argument("--shard-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -318,9 +421,11 @@ def main():
                                "reference path within 1e-9)",
                    "grid_points": nx * ny * nz, "tissue_cells": anat.ncell, "dt_ms": DT,
                    "mode": "fp64_fma",
-                   "launch": {"waves": f64["launch"][0], "block": f64["launch"][1],
-                              "occupancy": f64["launch"][2]},
-                   "graph_steps": args.graph},
+                   "launch": launch_json(f64["launch"]),
+                   "graph_steps": args.graph,
+                   "setup_steps_before_warmup": f64["setup_steps"]},
+        "autotune": {"fp64_fma": tune_json(f64["tune"]), "mixed_fused": tune_json(mx["tune"]),
+                     "columns": ["block", "occupancy cap", "tile order", "ms per step"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic("engine_f64_fma_shell256x256x512"),
@@ -339,30 +444,29 @@ def main():
         f"{ceil['inplace_GBps']:.0f} GB/s; the FP64 step streams {streamed:.0f} GB/s")
     mms = mx["ms"] / mx["steps"]
     both = (f64["act"] >= 0) & (mx["act"] >= 0)
+    dropin = res["fp64_unfused"]["rate"] if "fp64_unfused" in res else None
+    refr = res["fp64_fused"]["rate"] if "fp64_fused" in res else None
     out["mixed_fused"] = {
         "value": mx["rate"], "unit": "Gcell-steps/s", "steps": mx["steps"], "ms_per_step": mms,
-        "speedup_vs_fp64": mx["rate"] / f64["rate"],
-        # against the plain FP64 drop-in path: the reference's loop over the
-        # drop-in Reaction / Diffusion objects
-        "speedup_vs_fp64_dropin": (mx["rate"] / res["fp64_unfused"]["rate"]
-                                   if "fp64_unfused" in res else None),
-        "launch": {"waves": mx["launch"][0], "block": mx["launch"][1],
-                   "occupancy": mx["launch"][2]},
+        # one definition: mixed cell-steps/s over the FP64 path's, same anatomy
+        # and step; the fused FP64 step is the headline ratio
+        "speedup_vs_fp64_fused_fma": mx["rate"] / f64["rate"],
+        "speedup_vs_fp64_fused_reference_rounding": mx["rate"] / refr if refr else None,
+        "speedup_vs_fp64_dropin": mx["rate"] / dropin if dropin else None,
+        "launch": launch_json(mx["launch"]),
         "algorithmic_bytes_per_cell_step": MIXED_BYTES,
         "achieved_GBps": MIXED_BYTES * anat.ncell / (mms * 1e-3) / 1e9,
         "streamed_bytes_per_cell_step": MIXED_STREAMED,
         "streamed_GBps": MIXED_STREAMED * anat.ncell / (mms * 1e-3) / 1e9,
         "traffic": load_traffic("engine_mixed_shell256x256x512"),
         "valu": valu("engine_mixed_shell256x256x512", anat.ncell, mms)}
-    if f64["steps"] == mx["steps"]:
-        out["mixed_fused"]["error_vs_fp64_at_end"] = {
-            "t_ms": (args.warmup + f64["steps"]) * DT,
-            "linf_vm_mV": float(np.abs(mx["vm"] - f64["vm"]).max()),
-            "activation_max_abs_ms": float(np.abs(mx["act"] - f64["act"])[both].max()),
-            # the front is still moving at t_ms: a cell it reaches in the last step
-            # of one run may not be activated yet in the other
-            "activated_fp64": int((f64["act"] >= 0).sum()),
-            "activated_in_one_run_only": int(((f64["act"] >= 0) != (mx["act"] >= 0)).sum())}
+    if args.accuracy_steps > 0:
+        acc = accuracy(np, hip, slab, anat.ncell, args.accuracy_steps, args.graph)
+        out["mixed_fused"].update(
+            linf_vm_mV=acc["linf_vm_mV"], activation_max_abs_ms=acc["activation_max_abs_ms"],
+            apd90_rel_max=acc["apd90_rel_max"], error_vs_fp64=acc)
+        log(f"accuracy over {acc['t_ms']:.0f} ms: L-inf Vm {acc['linf_vm_mV']:.3e} mV, "
+            f"activation {acc['activation_max_abs_ms']} ms, APD90 rel {acc['apd90_rel_max']}")
     out["range_check"] = {"fp64_out_of_range": (f64["range"] or {}).get("count", 0),
                           "mixed_out_of_range": (mx["range"] or {}).get("count", 0)}
     if "fp64_fused" in res:
@@ -370,8 +474,7 @@ def main():
         out["fp64_reference_rounding"] = {
             "value": ex["rate"], "unit": "Gcell-steps/s", "steps": ex["steps"],
             "ms_per_step": ex["ms"] / ex["steps"],
-            "launch": {"waves": ex["launch"][0], "block": ex["launch"][1],
-                       "occupancy": ex["launch"][2]},
+            "launch": launch_json(ex["launch"]),
             "note": "no FMA anywhere: bit for bit the unfused step's rounding",
             "mixed_speedup": mx["rate"] / ex["rate"]}
     if "fp64_unfused" in res:
@@ -404,8 +507,17 @@ def main():
             log(f"serving, {args.serve_cap} cells in flight at most: p50 "
                 f"{out['serving_capped']['p50_latency_s']:.2f} s, p99 "
                 f"{out['serving_capped']['p99_latency_s']:.2f} s")
+    if args.steady_requests > 0:
+        # the same request mix below the GPU's capacity (about 1.8 req/s of
+        # this mix at ~19 Gcell-steps/s): latency is admission and shared
+        # steps, not one overloaded batch's makespan
+        out["serving_steady"] = serving(hip, W, args.steady_requests, args.steady_rate,
+                                        args.serve_hbm_gb, args.graph)
+        log(f"serving at {args.steady_rate} req/s: p50 "
+            f"{out['serving_steady']['p50_latency_s']:.2f} s, p99 "
+            f"{out['serving_steady']['p99_latency_s']:.2f} s")
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle on host cores)")
+        log("cpu baselines (oracle, and the reference's reaction kernels, on host cores)")
         out["cpu_baseline"] = cpu_baseline()
     else:
         out["cpu_baseline"] = None

@@ -231,18 +231,19 @@ int cdh_batch_set_graph(cdh_batch *b, int steps)
    return guard([&] { engine(b).set_graph_steps(steps); });
 }
 
-int cdh_batch_set_launch(cdh_batch *b, int waves, int block, int occupancy)
+int cdh_batch_set_launch(cdh_batch *b, int waves, int block, int occupancy, int tile_order)
 {
-   return guard([&] { engine(b).set_launch(LaunchCfg{waves, block, occupancy}); });
+   return guard([&] { engine(b).set_launch(LaunchCfg{waves, block, occupancy, tile_order}); });
 }
 
-int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block, int *occupancy)
+int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block, int *occupancy, int *tile_order)
 {
    return guard([&] {
       const LaunchCfg c = engine(b).launch();
       if (waves) *waves = c.waves;
       if (block) *block = c.block;
       if (occupancy) *occupancy = c.occ;
+      if (tile_order) *tile_order = c.xcd;
    });
 }
 
@@ -255,10 +256,11 @@ int cdh_batch_autotune(cdh_batch *b, int steps, double *best_ms, double *table, 
       if (best_ms) *best_ms = best;
       int k = 0;
       for (; k < (int)t.size() && k < max_rows && table; k++) {
-         table[4 * k] = t[k].first.waves;
-         table[4 * k + 1] = t[k].first.block;
-         table[4 * k + 2] = t[k].first.occ;
-         table[4 * k + 3] = t[k].second;
+         table[5 * k] = t[k].first.waves;
+         table[5 * k + 1] = t[k].first.block;
+         table[5 * k + 2] = t[k].first.occ;
+         table[5 * k + 3] = t[k].first.xcd;
+         table[5 * k + 4] = t[k].second;
       }
       if (nrows) *nrows = (int)t.size();
    });

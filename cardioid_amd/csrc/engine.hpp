@@ -93,7 +93,9 @@ struct EngineArgs {
                                 // they are the one range tile_base .. tile_base + ntile - 1
    int tile_base;
    int ntile;                   // live tiles
-   int groups_per_xcd;          // workgroup tiles per XCD
+   int groups_per_xcd;          // workgroups per XCD (grid = 8 * groups_per_xcd)
+   int xcd_chunk_log2;          // tile order: -1 contiguous per XCD, else runs of 2^this
+                                // workgroups dealt to the XCDs in turn
    double dt;
    uint32_t gate_mask;          // bits 0..12 set; see gates64 and the kernels
    int sensors;                 // bit 0: activation times, bit 1: APD90
@@ -203,6 +205,22 @@ __device__ __forceinline__ void engine_range(const EngineArgs &a, const SlabDev 
    const bool out = live && (V > S.vm_max || V < S.vm_min);
    const uint64_t m = __ballot(out);
    if (m) range_report(a.ranges + sl, m, lane, S.loop[parity], i, V, dVR, dVD);
+}
+
+// The tile group (workgroup-sized run of tiles) of this workgroup.
+// Workgroups are dealt round robin to the 8 XCDs (workgroup w on XCD w mod
+// 8), and each XCD has its own L2.  Contiguous order (xcd_chunk_log2 < 0):
+// XCD x serves the x-th eighth of the tile groups, so that its L2 holds the
+// box rows its cells read.  Chunked order: runs of G = 2^xcd_chunk_log2
+// consecutive tile groups go to the XCDs in turn.  Both cover every group
+// once; the grid may hold a few workgroups past the last group, which exit.
+__device__ __forceinline__ int engine_tile_group(const EngineArgs &a)
+{
+   const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
+   if (a.xcd_chunk_log2 < 0) return xcd * a.groups_per_xcd + r;
+   const int sh = a.xcd_chunk_log2;
+   const int c = r >> sh;
+   return (((c << 3) + xcd) << sh) + (r - (c << sh));
 }
 
 // Buffer resource over a box so that (box index)*8 is the VGPR offset and

@@ -65,8 +65,9 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    // The states are stored once the range check and the stimulus have read
    // the slab's descriptor (reads after a store become vector loads that
    // wait for it) in the reference-rounding instance; the FMA instance keeps
-   // its stores first.  Measured on one MI355X: 1.532 against 1.550 ms per step
-   // with reference rounding, 1.452 against 1.434 ms with FMA.
+   // its stores first.  Measured back to back on one MI355X (20.1 M-cell
+   // shell): 1.577 against 1.594 ms per step with reference rounding; the
+   // FMA instance ran 1.3 % slower with the later stores.
    if constexpr (FMA) {
 #pragma unroll
       for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
@@ -86,20 +87,17 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 
 }  // namespace
 
-// Workgroups are dealt round-robin to the 8 XCDs; workgroup w serves a tile
-// group in the contiguous range of XCD (w mod 8), so that each XCD's L2
-// holds the box rows its cells read.  The grid is 8 * groups_per_xcd and
-// covers the live slabs' tiles only: tiles of removed slabs cost nothing.
+// Workgroups map to tiles in the XCD-aware order of engine_tile_group.  The
+// grid covers the live slabs' tiles only: tiles of removed slabs cost nothing.
 // When the live tiles are one range (one anatomy, a batch without holes)
 // the tile is the index plus a base, with no list to read first.
 template <int WAVES, bool FMA>
 __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
 {
-   const int xcd = blockIdx.x & 7;
    const int wpg = blockDim.x >> 6;
    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
-   const int j = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
+   const int j = engine_tile_group(a) * wpg + wave;
    if (j >= a.ntile) return;
    const int blk = a.tiles ? (int)a.tiles[j] : a.tile_base + j;
    const int sl = a.blk_slab[blk];

@@ -143,13 +143,18 @@ void download(const T *dev, T *host, size_t count)
 
 }  // namespace
 
+// The autotuner's axes, each of which changes what runs: the workgroup size,
+// the occupancy cap, and the order in which tiles meet the XCDs (contiguous
+// eighths, or runs of 1 or 16 workgroups dealt in turn).  The kernel
+// instances' register targets are not an axis: with the current kernels the
+// two targets of each mode compile to the same register count.
 std::vector<LaunchCfg> Engine::candidates(int mode)
 {
    std::vector<LaunchCfg> c;
    const int w0 = min_waves(mode);
-   for (int w = w0; w <= w0 + 1; w++)
-      for (int b : {64, 128, 256})
-         for (int o : {0, 2, 3}) c.push_back(LaunchCfg{w, b, o});
+   for (int b : {64, 128, 256})
+      for (int o : {0, 3})
+         for (int x : {-1, 0, 4}) c.push_back(LaunchCfg{w0, b, o, x});
    return c;
 }
 
@@ -172,31 +177,79 @@ void Engine::track(DevBuf<T> &b, size_t count)
    bytes_ += (int64_t)(count * sizeof(T));
 }
 
+// Pinned staging for host-to-device copies on the engine's stream.  Small
+// copies (the descriptors and lists of one add, up to kSmallPut bytes each)
+// share a block, each taking the next 256-byte-aligned piece of it; a large
+// copy takes a block of its own.  A block's event is recorded after every
+// copy out of it, so once the event has completed every copy out of the
+// block is done and the whole block is free again.
 char *Engine::pinned(size_t bytes, hipEvent_t *ev)
 {
-   PinBlock *best = nullptr;
-   for (PinBlock &b : pins_) {
-      if (b.busy && hipEventQuery(b.ev) == hipSuccess) b.busy = false;
-      if (!b.busy && b.cap >= bytes && (!best || b.cap < best->cap)) best = &b;
-   }
-   if (!best) {
-      const size_t mib = size_t(1) << 20;
-      const size_t cap = (bytes + mib - 1) / mib * mib;
-      if (pinned_bytes_ + cap > kPinnedLimit) return nullptr;
-      PinBlock b;
-      b.cap = cap;
-      CDH_HIP(hipHostMalloc((void **)&b.p, cap, hipHostMallocDefault));
-      if (hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
-         (void)hipHostFree(b.p);
-         throw Error(kErrHip, "engine: hipEventCreateWithFlags failed");
+   for (PinBlock &b : pins_)
+      if (b.busy && hipEventQuery(b.ev) == hipSuccess) {
+         b.busy = false;
+         b.used = 0;
       }
-      pins_.push_back(b);
-      pinned_bytes_ += cap;
-      best = &pins_.back();
+   const size_t mib = size_t(1) << 20;
+   const bool small = bytes <= kSmallPut;
+   const size_t need = (bytes + 255) / 256 * 256;
+   PinBlock *blk = nullptr;
+   if (small && fill_ >= 0 && pins_[fill_].used + need <= pins_[fill_].cap) blk = &pins_[fill_];
+   if (!blk) {
+      for (PinBlock &b : pins_)
+         if (!b.busy && b.cap >= need && (!blk || b.cap < blk->cap)) blk = &b;
+      if (!blk) {
+         const size_t cap = std::max(mib, (need + mib - 1) / mib * mib);
+         if (pinned_bytes_ + cap > kPinnedLimit) return nullptr;
+         PinBlock b;
+         b.cap = cap;
+         CDH_HIP(hipHostMalloc((void **)&b.p, cap, hipHostMallocDefault));
+         if (hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipHostFree(b.p);
+            throw Error(kErrHip, "engine: hipEventCreateWithFlags failed");
+         }
+         pins_.push_back(b);
+         pinned_bytes_ += cap;
+         blk = &pins_.back();
+      }
+      fill_ = small ? (int)(blk - pins_.data()) : (fill_ == (int)(blk - pins_.data()) ? -1 : fill_);
    }
-   best->busy = true;
-   *ev = best->ev;
-   return best->p;
+   char *p = blk->p + blk->used;
+   blk->used = small ? blk->used + need : blk->cap;
+   blk->busy = true;
+   *ev = blk->ev;
+   return p;
+}
+
+// Keeps at most kPinnedKeep bytes of idle staging: blocks whose copies are
+// done are freed, largest first, above that mark.
+void Engine::trim_pinned()
+{
+   if (pinned_bytes_ <= kPinnedKeep) return;
+   std::vector<size_t> idle;
+   for (size_t k = 0; k < pins_.size(); k++) {
+      PinBlock &b = pins_[k];
+      if (b.busy && hipEventQuery(b.ev) == hipSuccess) {
+         b.busy = false;
+         b.used = 0;
+      }
+      if (!b.busy) idle.push_back(k);
+   }
+   std::sort(idle.begin(), idle.end(),
+             [&](size_t a, size_t b) { return pins_[a].cap > pins_[b].cap; });
+   std::vector<bool> drop(pins_.size(), false);
+   for (size_t k : idle) {
+      if (pinned_bytes_ <= kPinnedKeep) break;
+      (void)hipHostFree(pins_[k].p);
+      (void)hipEventDestroy(pins_[k].ev);
+      pinned_bytes_ -= pins_[k].cap;
+      drop[k] = true;
+   }
+   std::vector<PinBlock> kept;
+   for (size_t k = 0; k < pins_.size(); k++)
+      if (!drop[k]) kept.push_back(pins_[k]);
+   pins_.swap(kept);
+   fill_ = -1;
 }
 
 template <class T>
@@ -238,6 +291,7 @@ void Engine::refresh_tiles()
 
 void Engine::retire_staging()
 {
+   trim_pinned();
    if (staging_.empty()) return;
    if (hipEventQuery(staging_ev_) == hipSuccess) staging_.clear();
 }
@@ -528,7 +582,15 @@ EngineArgs Engine::args(int parity) const
    a.ntile = (int)nlist_;
    const int wpg = cfg_.block / kBlk;
    const int ngroup = (a.ntile + wpg - 1) / wpg;
-   a.groups_per_xcd = (ngroup + 7) / 8;
+   a.xcd_chunk_log2 = cfg_.xcd;
+   if (cfg_.xcd < 0) {
+      a.groups_per_xcd = (ngroup + 7) / 8;
+   } else {
+      // whole rounds of 8 runs of G groups: every group index below
+      // 8 * G * rounds is reached, the ones past ngroup exit
+      const int G = 1 << cfg_.xcd;
+      a.groups_per_xcd = (ngroup + 8 * G - 1) / (8 * G) * G;
+   }
    a.dt = dt_;
    a.gate_mask = 0x1FFFu;
    a.sensors = sensors_;
@@ -563,7 +625,7 @@ void Engine::upload_halo()
    sync();
    nhalo_ = (int64_t)all.size() / 2;
    if (halo_.n < all.size()) halo_.alloc(all.size());
-   if (!all.empty()) halo_.upload(all.data(), all.size());
+   put(halo_.p, all);   // in stream order, before any later step reads it
    ++halo_version_;
 }
 
@@ -589,11 +651,11 @@ int64_t Engine::link_halo(int dst, int src)
    const int64_t np = (int64_t)l.pairs.size() / 2;
    // the remote points take src's current Vm now, then after every step
    DevBuf<uint32_t> once(l.pairs.size());
-   once.upload(l.pairs.data(), l.pairs.size());
+   put(once.p, l.pairs);   // in stream order, before the kernel that reads it
    hipLaunchKernelGGL(k_halo, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, stream_, np,
                       once.p, box_[parity_].p);
    CDH_LAUNCH_CHECK();
-   sync();
+   sync();                 // `once` is freed on return
    links_.push_back(std::move(l));
    upload_halo();
    return np;
@@ -654,6 +716,7 @@ void Engine::set_launch(const LaunchCfg &c)
    CDH_REQUIRE(c.block == 64 || c.block == 128 || c.block == 256,
                "engine: workgroup size must be 64, 128 or 256");
    CDH_REQUIRE(c.occ >= 0 && c.occ <= 8, "engine: occupancy cap must be 0..8");
+   CDH_REQUIRE(c.xcd >= -1 && c.xcd <= 10, "engine: tile order must be -1 or 0..10");
    cfg_ = c;
    drop_graph();
 }
@@ -668,6 +731,7 @@ void Engine::run(int nsteps)
          if (!graph_ || graph_ntile_ != (int)nlist_ || graph_direct_ != tiles_direct_ ||
              graph_base_ != tile_base_ || graph_cfg_.waves != cfg_.waves ||
              graph_cfg_.block != cfg_.block || graph_cfg_.occ != cfg_.occ ||
+             graph_cfg_.xcd != cfg_.xcd ||
              graph_halo_ != halo_version_)
             rebuild_graph();
          CDH_HIP(hipGraphLaunch(graph_, stream_));

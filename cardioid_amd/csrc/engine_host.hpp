@@ -42,6 +42,8 @@ struct LaunchCfg {
    int block = 256;   // threads per workgroup: 64, 128 or 256
    int occ = 0;       // waves per SIMD allowed at run time (0: no cap), set
                       // by reserving LDS per workgroup
+   int xcd = -1;      // tile order over the XCDs: -1 contiguous eighths, k >= 0
+                      // runs of 2^k workgroups dealt in turn (engine_tile_group)
 };
 
 class Engine {
@@ -135,6 +137,7 @@ class Engine {
    template <class T>
    void put(T *dst, const std::vector<T> &src);
    char *pinned(size_t bytes, hipEvent_t *ev);
+   void trim_pinned();
    template <class T>
    void track(DevBuf<T> &b, size_t count);
    void check_slab(int id) const;
@@ -175,11 +178,15 @@ class Engine {
    struct PinBlock {
       char *p = nullptr;
       size_t cap = 0;
-      hipEvent_t ev = nullptr;
+      size_t used = 0;             // bytes handed out since the block was last idle
+      hipEvent_t ev = nullptr;     // recorded after every copy out of the block
       bool busy = false;
    };
    static constexpr size_t kPinnedLimit = size_t(1) << 30;
+   static constexpr size_t kPinnedKeep = size_t(64) << 20;   // idle staging kept
+   static constexpr size_t kSmallPut = size_t(64) << 10;     // copies that share a block
    std::vector<PinBlock> pins_;
+   int fill_ = -1;                 // the block small copies are taking pieces of
    size_t pinned_bytes_ = 0;
    SlabSetupScratch setup_;        // device set-up buffers, kept across slabs
    // the live slabs' tiles in arena order, the step kernel's grid: rebuilt

@@ -130,11 +130,10 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
 template <int WAVES>
 __global__ __launch_bounds__(256, WAVES) void k_engine_mixed(EngineArgs a)
 {
-   const int xcd = blockIdx.x & 7;
    const int wpg = blockDim.x >> 6;
    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
-   const int j = (xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * wpg + wave;
+   const int j = engine_tile_group(a) * wpg + wave;
    if (j >= a.ntile) return;
    const int blk = a.tiles ? (int)a.tiles[j] : a.tile_base + j;
    const int sl = a.blk_slab[blk];

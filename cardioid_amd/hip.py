@@ -199,8 +199,8 @@ def lib():
         L.cdh_batch_run_timed.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_sync.argtypes = [C.c_void_p]
         L.cdh_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
-        L.cdh_batch_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
-        L.cdh_batch_get_launch.argtypes = [C.c_void_p, ip, ip, ip]
+        L.cdh_batch_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.cdh_batch_get_launch.argtypes = [C.c_void_p, ip, ip, ip, ip]
         L.cdh_batch_autotune.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_int, ip]
         L.cdh_batch_get_vm.argtypes = [C.c_void_p, C.c_int, dp]
         L.cdh_batch_get_state.argtypes = [C.c_void_p, C.c_int, dp]
@@ -579,25 +579,28 @@ class Batch:
     def set_graph(self, steps: int):
         check(lib().cdh_batch_set_graph(self.h, int(steps)), "cdh_batch_set_graph")
 
-    def set_launch(self, waves: int, block: int, occupancy: int = 0):
-        check(lib().cdh_batch_set_launch(self.h, waves, block, occupancy), "cdh_batch_set_launch")
+    def set_launch(self, waves: int, block: int, occupancy: int = 0, tile_order: int = -1):
+        check(lib().cdh_batch_set_launch(self.h, waves, block, occupancy, tile_order),
+              "cdh_batch_set_launch")
 
     def launch(self):
-        """(register target, workgroup size, occupancy cap)"""
-        w, b, o = C.c_int(), C.c_int(), C.c_int()
-        check(lib().cdh_batch_get_launch(self.h, C.byref(w), C.byref(b), C.byref(o)),
+        """(register target, workgroup size, occupancy cap, tile order)"""
+        w, b, o, x = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().cdh_batch_get_launch(self.h, C.byref(w), C.byref(b), C.byref(o), C.byref(x)),
               "cdh_batch_get_launch")
-        return w.value, b.value, o.value
+        return w.value, b.value, o.value, x.value
 
     def autotune(self, steps: int = 10):
-        """(best ms per step, [(waves, block, occupancy, ms per step), ...])"""
+        """(best ms per step, [(waves, block, occupancy, tile order, ms per
+        step), ...])"""
         best = C.c_double()
-        table = np.zeros(4 * 64)
+        table = np.zeros(5 * 64)
         nrows = C.c_int()
         check(lib().cdh_batch_autotune(self.h, int(steps), C.byref(best), ptr(table), 64,
                                        C.byref(nrows)), "cdh_batch_autotune")
-        rows = [(int(table[4 * k]), int(table[4 * k + 1]), int(table[4 * k + 2]),
-                 float(table[4 * k + 3])) for k in range(min(nrows.value, 64))]
+        rows = [(int(table[5 * k]), int(table[5 * k + 1]), int(table[5 * k + 2]),
+                 int(table[5 * k + 3]), float(table[5 * k + 4]))
+                for k in range(min(nrows.value, 64))]
         return best.value, rows
 
     def vm(self, sid: int) -> np.ndarray:

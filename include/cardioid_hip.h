@@ -402,14 +402,16 @@ int cdh_batch_sync(cdh_batch *b);
 /* steps per captured HIP graph (even; 0 launches each step directly) */
 int cdh_batch_set_graph(cdh_batch *b, int steps);
 /* launch configuration: the kernel instance's register target (waves/SIMD:
- * FP64 2..3, mixed 3..4), the workgroup size (64, 128, 256) and a cap on
- * the waves per SIMD resident at run time (0: none; applied by reserving
- * LDS per workgroup) */
-int cdh_batch_set_launch(cdh_batch *b, int waves, int block, int occupancy);
-int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block, int *occupancy);
+ * FP64 2..3, mixed 3..4), the workgroup size (64, 128, 256), a cap on the
+ * waves per SIMD resident at run time (0: none; applied by reserving LDS per
+ * workgroup) and the order in which tiles meet the 8 XCDs, each with its own
+ * L2 (-1: XCD x serves the x-th contiguous eighth of the tiles; k >= 0: runs
+ * of 2^k workgroups dealt to the XCDs in turn) */
+int cdh_batch_set_launch(cdh_batch *b, int waves, int block, int occupancy, int tile_order);
+int cdh_batch_get_launch(cdh_batch *b, int *waves, int *block, int *occupancy, int *tile_order);
 /* times every launch configuration for `steps` steps from the current state
  * (which is restored afterwards) and keeps the fastest; the table gets
- * (waves, block, occupancy, ms per step) rows, up to max_rows */
+ * (waves, block, occupancy, tile order, ms per step) rows, up to max_rows */
 int cdh_batch_autotune(cdh_batch *b, int steps, double *best_ms, double *table, int max_rows,
                        int *nrows);
 /* per-slab results, in the slab's input cell order */

@@ -41,7 +41,12 @@ struct ora_sim {
    int ngroup;
    int64_t **gidx;             /* member local indices per group */
    int64_t *gcount;
-   double **gVm, **gdV, **gstate; /* gathered per-group buffers */
+   int64_t *gstride;           /* per-group state row stride: count rounded up to 4 */
+   double **gVm, **gdV, **gstate; /* gathered per-group buffers, 64-byte aligned */
+   /* reaction kernels other than the oracle's own (ora_sim_set_reaction) */
+   ora_react_fn react;
+   void *react_ctx;
+   int react_threads, pointwise_threads;
    /* stimuli */
    int nstim;
    ora_box_stim *stim;
@@ -119,15 +124,26 @@ ora_sim *ora_sim_create(const ora_sim_params *p, int nstim, const ora_box_stim *
    s->gVm = (double **)calloc(s->ngroup, sizeof(double *));
    s->gdV = (double **)calloc(s->ngroup, sizeof(double *));
    s->gstate = (double **)calloc(s->ngroup, sizeof(double *));
+   s->gstride = (int64_t *)calloc(s->ngroup, sizeof(int64_t));
    for (int64_t c = 0; c < n; c++) s->gcount[p->celltype[c]]++;
    for (int g = 0; g < s->ngroup; g++) {
+      /* rows of 4-cell multiples from 64-byte aligned bases: the reference's
+       * v2 gate kernels load 4 doubles at a time from 32-byte aligned
+       * addresses (TT06Dev_Reaction.cc:97, :180-192) */
       int64_t m = s->gcount[g] ? s->gcount[g] : 1;
+      int64_t ms = (m + 3) / 4 * 4;
+      s->gstride[g] = ms;
       s->gidx[g] = (int64_t *)malloc(sizeof(int64_t) * m);
-      s->gVm[g] = (double *)malloc(sizeof(double) * m);
-      s->gdV[g] = (double *)malloc(sizeof(double) * m);
-      s->gstate[g] = (double *)malloc(sizeof(double) * ORA_NSTATE * m);
+      if (posix_memalign((void **)&s->gVm[g], 64, sizeof(double) * ms) ||
+          posix_memalign((void **)&s->gdV[g], 64, sizeof(double) * ms) ||
+          posix_memalign((void **)&s->gstate[g], 64, sizeof(double) * ORA_NSTATE * ms))
+         abort();
+      memset(s->gVm[g], 0, sizeof(double) * ms);
+      memset(s->gdV[g], 0, sizeof(double) * ms);
+      memset(s->gstate[g], 0, sizeof(double) * ORA_NSTATE * ms);
       s->gcount[g] = 0;
    }
+   s->react_threads = s->pointwise_threads = p->nthreads;
    for (int64_t c = 0; c < n; c++) {
       int g = p->celltype[c];
       s->gidx[g][s->gcount[g]++] = c;
@@ -184,6 +200,7 @@ void ora_sim_destroy(ora_sim *s)
       free(s->gidx[g]); free(s->gVm[g]); free(s->gdV[g]); free(s->gstate[g]);
    }
    free(s->gidx); free(s->gcount); free(s->gVm); free(s->gdV); free(s->gstate);
+   free(s->gstride);
    for (int k = 0; k < s->nstim; k++) free(s->stimList[k]);
    free(s->stimList); free(s->stimCount); free(s->stim);
    free(s->blockIndex); free(s->A); free(s->A0); free(s->VmBlock);
@@ -207,6 +224,7 @@ static void reaction(ora_sim *s)
    for (int g = 0; g < s->ngroup; g++) {
       const int64_t m = s->gcount[g];
       if (m == 0) continue;
+      const int64_t ms = s->gstride[g];
       const ora_celltype *ct = &p->celltypes[g];
       int64_t *ix = s->gidx[g];
       double *gV = s->gVm[g], *gd = s->gdV[g], *gs = s->gstate[g];
@@ -214,7 +232,7 @@ static void reaction(ora_sim *s)
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
       for (int64_t i = 0; i < m; i++) {
          gV[i] = s->Vm[ix[i]];
-         for (int k = 0; k < ORA_NSTATE; k++) gs[k * m + i] = s->state[k * s->n + ix[i]];
+         for (int k = 0; k < ORA_NSTATE; k++) gs[k * ms + i] = s->state[k * s->n + ix[i]];
       }
       /* The reference runs these loops on one thread over the whole group.
        * Per-cell results do not depend on the split except through the v2
@@ -222,42 +240,53 @@ static void reaction(ora_sim *s)
        * their gates) and the f gate's form (lanes 0..3 of each 12-cell group
        * below 12*floor(m/12)).  Chunks that start at multiples of 12 see
        * both rules exactly as the whole group does. */
-      int nthr = p->nthreads > 0 ? p->nthreads : 1;
+      int nthr = s->react_threads > 0 ? s->react_threads : 1;
       int64_t chunk = ((m + nthr - 1) / nthr + 11) / 12 * 12;
-#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+#pragma omp parallel for schedule(static) num_threads(nthr)
       for (int t = 0; t < nthr; t++) {
          int64_t b = (int64_t)t * chunk;
          if (b >= m) continue;
          int64_t e = b + chunk < m ? b + chunk : m;
          int cnt = (int)(e - b);
+         if (s->react) {
+            s->react(s->react_ctx, g, p->dt, cnt, gV + b, gs + b, ms, gd + b);
+            continue;
+         }
          const ora_scales *sc = NULL;
          if (p->scales) {
             /* per-cell scales: evaluate cell by cell */
             for (int64_t i = b; i < e; i++) {
                sc = &p->scales[ix[i]];
-               if (p->exact) ora_tt06_nongate_exact(p->dt, ct, sc, 1, gV + i, gs + i, m, gd + i);
-               else ora_tt06_nongate(&s->fits, p->dt, ct, sc, 1, gV + i, gs + i, m, gd + i);
+               if (p->exact) ora_tt06_nongate_exact(p->dt, ct, sc, 1, gV + i, gs + i, ms, gd + i);
+               else ora_tt06_nongate(&s->fits, p->dt, ct, sc, 1, gV + i, gs + i, ms, gd + i);
             }
          } else {
-            if (p->exact) ora_tt06_nongate_exact(p->dt, ct, NULL, cnt, gV + b, gs + b, m, gd + b);
-            else ora_tt06_nongate(&s->fits, p->dt, ct, NULL, cnt, gV + b, gs + b, m, gd + b);
+            if (p->exact) ora_tt06_nongate_exact(p->dt, ct, NULL, cnt, gV + b, gs + b, ms, gd + b);
+            else ora_tt06_nongate(&s->fits, p->dt, ct, NULL, cnt, gV + b, gs + b, ms, gd + b);
          }
          if (p->exact) {
-            ora_tt06_gates_exact(p->dt, ct->s_switch, p->jhTauSmooth, cnt, gV + b, gs + b, m);
+            ora_tt06_gates_exact(p->dt, ct->s_switch, p->jhTauSmooth, cnt, gV + b, gs + b, ms);
+         } else if (p->no_loop_bounds) {
+            ora_tt06_gates_v2_all(&s->fits, ct->s_switch, cnt, gV + b, gs + b, ms);
          } else {
-            /* chunk boundaries are multiples of 12: see above */
-            if (p->no_loop_bounds)
-               ora_tt06_gates_v2_all(&s->fits, ct->s_switch, cnt, gV + b, gs + b, m);
-            else
-               ora_tt06_gates_v2(&s->fits, ct->s_switch, cnt, gV + b, gs + b, m);
+            ora_tt06_gates_v2(&s->fits, ct->s_switch, cnt, gV + b, gs + b, ms);
          }
       }
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
       for (int64_t i = 0; i < m; i++) {
          s->dVmR[ix[i]] = gd[i];
-         for (int k = 0; k < ORA_NSTATE; k++) s->state[k * s->n + ix[i]] = gs[k * m + i];
+         for (int k = 0; k < ORA_NSTATE; k++) s->state[k * s->n + ix[i]] = gs[k * ms + i];
       }
    }
+}
+
+void ora_sim_set_reaction(ora_sim *s, ora_react_fn fn, void *ctx, int react_threads,
+                          int pointwise_threads)
+{
+   s->react = fn;
+   s->react_ctx = ctx;
+   s->react_threads = react_threads;
+   s->pointwise_threads = pointwise_threads;
 }
 
 /* APD90 bookkeeping of one cell after an evaluation at `time`.  phase 0:
@@ -299,7 +328,7 @@ void ora_apd_trace(const double *V, int nt, double t0, double dt, double vrest,
 static void sensors(ora_sim *s)
 {
    const double thr = s->p.threshold;
-#pragma omp parallel for schedule(static) num_threads(s->p.nthreads)
+#pragma omp parallel for schedule(static) num_threads(s->pointwise_threads)
    for (int64_t c = 0; c < s->n; c++) {
       double v = s->Vm[c];
       if (!s->act0[c] && v > 0) { s->act0[c] = 1; s->tact0[c] = s->time; }
@@ -333,7 +362,7 @@ void ora_sim_run(ora_sim *s, int nsteps)
 #pragma omp parallel for schedule(static) num_threads(s->p.nthreads)
       for (int64_t c = 0; c < n; c++) s->VmBlock[s->blockIndex[c]] = s->Vm[c];
       ora_fgr_calc(n, s->blockIndex, s->A, s->A0, s->off, s->VmBlock, s->p.diffusionScale, s->dVmD);
-#pragma omp parallel for schedule(static) num_threads(s->p.nthreads)
+#pragma omp parallel for schedule(static) num_threads(s->pointwise_threads)
       for (int64_t c = 0; c < n; c++) s->Vm[c] += dt * (s->dVmR[c] + s->dVmD[c] + s->iStim[c]);
       s->time += dt;
       ++s->loop;

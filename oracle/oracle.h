@@ -130,6 +130,12 @@ void ora_fgr_calc(int64_t ncell, const int64_t *blockIndex, const float *A,
 /* ---- whole omp-loop simulation ---- */
 typedef struct ora_sim ora_sim;
 
+/* A reaction group's update (non-gates, then gates) over n cells: Vm[n],
+ * state rows of `stride` doubles (64-byte aligned, stride a multiple of 4),
+ * dVdt[n]; group = the cell type's index. */
+typedef void (*ora_react_fn)(void *ctx, int group, double dt, int n, double *Vm, double *state,
+                             int64_t stride, double *dVdt);
+
 typedef struct {
    int nx, ny, nz;              /* global grid; gid = x + nx*(y + ny*z) */
    double dx, dy, dz, dt, diffusionScale;
@@ -160,6 +166,13 @@ void ora_sim_run(ora_sim *s, int nsteps);
 void ora_sim_get_vm(const ora_sim *s, double *out);
 void ora_sim_get_state(const ora_sim *s, double *out /* 19*ncell SoA */);
 void ora_sim_set_state(ora_sim *s, const double *in, const double *Vm);
+/* runs the reaction through fn instead of the oracle's own kernels, each
+ * group split over react_threads threads in 12-cell multiples (1: the
+ * reference's one thread per group); the integrator and sensor loops on
+ * pointwise_threads (1: the reference's serial host loops).  Used to time
+ * the reference's kernels in this loop. */
+void ora_sim_set_reaction(ora_sim *s, ora_react_fn fn, void *ctx, int react_threads,
+                          int pointwise_threads);
 /* restart: states and Vm (NULL keeps them), clock, fresh sensors */
 void ora_sim_restart(ora_sim *s, const double *in, const double *Vm, int loop, double time);
 /* first time Vm > 0 (ActivationTimeSensor), -1 if never */

@@ -10,6 +10,7 @@
  * then call update_nonGate and TT06Func::updateGateFast / the v2 kernels.
  */
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <string>
@@ -143,6 +144,38 @@ void ref_gate_v2(double dt, int eq, int n, double *Vm, double *g)
 }
 
 double ref_exact_func(int i, double V, int jhTauSmooth) { init_once(); return exact_func(i, jhTauSmooth)(V, NULL); }
+
+/* Per-group cell types for ref_react_group (one reaction object per cell
+ * type, ReactionManager.cc:184-261). */
+static CellTypeParms g_group_ct[8];
+
+void ref_set_group_celltype(int g, int s_switch, double P_NaK, double g_Ks, double g_Kr,
+                            double g_to, double g_NaL)
+{
+   if (g < 0 || g >= 8) return;
+   CellTypeParms &c = g_group_ct[g];
+   c.cellType = g;
+   c.s_switch = s_switch;
+   c.P_NaK = P_NaK; c.g_Ks = g_Ks; c.g_Kr = g_Kr; c.g_to = g_to; c.g_NaL = g_NaL;
+   c.minK_i = c.maxK_i = c.midK_i = 1.0;
+   c.minNa_i = c.maxNa_i = c.midNa_i = 1.0;
+}
+
+/* TT06Dev_Reaction::calc (TT06Dev_Reaction.cc:393-407) in configuration B on
+ * one reaction group's SoA state: update_nonGate, then updateGateFast (the
+ * v2 kernels).  The signature is the oracle loop's reaction hook
+ * (ora_react_fn), so the loop can time the reference's own kernels. */
+void ref_react_group(void *ctx, int group, double dt, int n, double *Vm, double *state,
+                     int64_t stride, double *dVdt)
+{
+   (void)ctx;
+   double *st[19];
+   for (int k = 0; k < 19; k++) st[k] = state + (size_t)k * stride;
+   CellTypeParms *ct = &g_group_ct[group];
+   update_nonGate((void *)g_fit, dt, ct, n, Vm, 0, st, dVdt);
+   TT06Func::WORK work = {0, n, 0, 12, NULL};
+   TT06Func::updateGateFast(dt, n, ct->s_switch, Vm, 0, &st[gateOffset], g_fit, work);
+}
 
 /* One row of TT06Func::getStandardCellTypes (TT06Func.cc:421-612):
  * parms = {P_NaK, g_Ks, g_Kr, g_to, g_NaL, Vm}, state[19] by each STATE's

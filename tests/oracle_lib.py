@@ -107,6 +107,7 @@ def oracle():
             getattr(L, fn).argtypes = [C.c_void_p, dp]
         L.ora_sim_set_state.argtypes = [C.c_void_p, dp, dp]
         L.ora_sim_restart.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_double]
+        L.ora_sim_set_reaction.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
         L.ora_sim_get_t90.argtypes = [C.c_void_p, dp]
         L.ora_apd_trace.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     dp, dp]
@@ -139,6 +140,7 @@ def ref():
         L.ref_exact_func.argtypes = [C.c_int, C.c_double, C.c_int]
         L.ref_fgr_f2.argtypes = [C.c_uint, ip, dp]
         L.ref_celltype_standard.argtypes = [C.c_char_p, ip, dp, dp]
+        L.ref_set_group_celltype.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5
         L.ref_celltype_names.argtypes = [C.c_char_p, C.c_int]
         L.ref_fibre_conductivity.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ref_init()
@@ -213,6 +215,7 @@ class OracleSim:
         from cardioid_amd.fitfile import read_fit_file
         O = oracle()
         fs = fits or read_fit_file()
+        self.fits, self.dt = fs, dt
         self.ofs = make_fitset(fs)
         self.cts = (OraCelltype * len(celltypes))(*[celltype(n) for n in celltypes])
         self.gid = np.ascontiguousarray(anat.gid, dtype=np.int64)
@@ -250,6 +253,20 @@ class OracleSim:
         out = np.zeros(NSTATE * self.n)
         oracle().ora_sim_get_state(self.h, ptr(out))
         return out.reshape(NSTATE, self.n)
+
+    def use_reference_reaction(self, react_threads=1, pointwise_threads=1):
+        """Run the reaction through the reference's own kernels
+        (update_nonGate and the v2 gate kernels, compiled from source into
+        oracle/_ref) instead of the oracle's restatement: react_threads = 1
+        is the reference's one thread per reaction group, pointwise_threads =
+        1 its serial integrator and sensor loops.  Configuration B only."""
+        R = ref_load(self.fits, self.dt, exact=False, jhTauSmooth=1)
+        for g, ct in enumerate(self.cts):
+            R.ref_set_group_celltype(g, ct.s_switch, ct.P_NaK, ct.g_Ks, ct.g_Kr, ct.g_to,
+                                     ct.g_NaL)
+        fn = C.cast(R.ref_react_group, C.c_void_p)
+        oracle().ora_sim_set_reaction(self.h, fn, None, int(react_threads),
+                                      int(pointwise_threads))
 
     def restart(self, state, vm, loop, time):
         """States [19][n] and Vm, the clock, and fresh sensors, as the

@@ -202,6 +202,30 @@ def test_threaded_oracle_equals_its_serial_run():
 
 
 @needs_ref
+@pytest.mark.parametrize("threads", [(1, 1), (6, 6)])
+def test_loop_with_the_reference_reaction_equals_the_oracle(threads):
+    """The omp loop with the reaction run by the reference's own kernels
+    (one thread per group as the reference runs it, or split in 12-cell
+    multiples) equals the oracle's loop bit for bit: Vm, all 19 states and
+    the activation times, over three cell types."""
+    from cardioid_amd import workloads as W
+    from oracle_lib import OracleSim
+    a, depth = W.ellipsoid_shell(20, 18, 26, seed=5)
+    stims = W.endocardial_point_stimuli(a, depth, 2, seed=5)
+    cts = ["endoCellML", "midCellML", "epiCellML"]
+    own = OracleSim(a, cts, DT, 714.2857143, stims, nthreads=6)
+    refk = OracleSim(a, cts, DT, 714.2857143, stims, nthreads=6)
+    refk.use_reference_reaction(*threads)
+    own.run(300)
+    refk.run(300)
+    assert np.array_equal(own.vm(), refk.vm())
+    assert np.array_equal(own.state(), refk.state())
+    for x, y in zip(own.activation(), refk.activation()):
+        assert np.array_equal(x, y)
+    assert (own.activation()[1] >= 0).any()
+
+
+@needs_ref
 def test_each_v2_gate_bitwise_over_group_sizes():
     """Group sizes cover every split of the 8/12/16-wide main loops, the
     4-wide tail loop and the unvisited remainder."""
