@@ -69,6 +69,9 @@ def main(argv=None):
     ap.add_argument("--cmd-write", default="")
     ap.add_argument("--note", default="")
     ap.add_argument("--copies", type=int, default=3)
+    ap.add_argument("--launches-per-step", type=int, default=1,
+                    help="launches of the kernel that together make one step (the drop-in "
+                         "reaction: one per cell type); per-step figures sum them")
     ap.add_argument("--valu", help="rocprofv3 --pmc run of SQ_WAVES SQ_INSTS_VALU "
                     "SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES GRBM_GUI_ACTIVE GRBM_COUNT")
     ap.add_argument("--cmd-valu", default="")
@@ -81,7 +84,8 @@ def main(argv=None):
 
     rows = trace_rows(a.trace)
     k = [r for r in rows if a.kernel in r[0]][0]
-    avg_us = k[3]
+    per = a.launches_per_step
+    avg_us = k[3] * per          # one step's launches
     lines = [f"# {a.title}", ""]
     if a.note:
         lines += [a.note, ""]
@@ -91,12 +95,15 @@ def main(argv=None):
     for r in rows[:6]:
         lines.append(f"| `{r[0][:90]}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.2f} |")
     achieved = a.bytes * a.cells / (avg_us * 1e-6) / 1e9
-    lines += ["", f"Average duration of `{a.kernel}`: {avg_us:.1f} us per launch (one step of "
+    what = ("per launch (one step of" if per == 1 else
+            f"per step ({per} launches of {k[3]:.1f} us on average, together one step of")
+    lines += ["", f"Average duration of `{a.kernel}`: {avg_us:.1f} us {what} "
               f"{a.cells} tissue cells) = {a.cells / avg_us / 1e3:.3f} Gcell-steps/s; at "
               f"{a.bytes:g} algorithmic bytes per cell-step that is {achieved:.0f} GB/s "
               f"({achieved / 8000:.3f} of 8 TB/s).", ""]
     out = {"kernel": k[0], "workload": a.workload, "tissue_cells": a.cells,
-           "average_us": avg_us, "algorithmic_bytes_per_cell_step": a.bytes}
+           "average_us": avg_us, "algorithmic_bytes_per_cell_step": a.bytes,
+           "launches_per_step": per}
     if a.fetch and a.write:
         f = counter_means(a.fetch, "FETCH_SIZE")
         w = counter_means(a.write, "WRITE_SIZE")
@@ -105,15 +112,15 @@ def main(argv=None):
         ws = a.copies * COPY_BYTES / (cw[2] * 1024)
         fk, nf, _ = f[_pick(f, a.kernel)]
         wk, nw, _ = w[_pick(w, a.kernel)]
-        fb, wb = fk * 1024 * fs, wk * 1024 * ws
-        lines += ["## HBM traffic per launch (FETCH_SIZE, WRITE_SIZE)", "",
+        fb, wb = fk * 1024 * fs * per, wk * 1024 * ws * per
+        lines += ["## HBM traffic per step (FETCH_SIZE, WRITE_SIZE)", "",
                   "Commands (one counter per run):", "", f"    {a.cmd_fetch}", f"    {a.cmd_write}",
                   "", f"Calibration: the {a.copies} device-to-device copies of 512 MiB at the start "
                   f"of the same runs ({cf[1]} dispatches of `{COPY_KERNEL}`) read FETCH_SIZE = "
                   f"{cf[2] / a.copies:.1f} KiB and WRITE_SIZE = {cw[2] / a.copies:.1f} KiB per copy, "
                   f"so bytes read = FETCH_SIZE x 1024 x {fs:.4f} and bytes written = WRITE_SIZE x "
                   f"1024 x {ws:.4f}.", "",
-                  "| | raw (KiB/launch) | launches | scaled bytes/launch | bytes per tissue cell-step |",
+                  "| | raw (KiB/launch) | launches | scaled bytes/step | bytes per tissue cell-step |",
                   "|---|---|---|---|---|",
                   f"| FETCH_SIZE | {fk:.1f} | {nf} | {fb:.4e} | {fb / a.cells:.1f} |",
                   f"| WRITE_SIZE | {wk:.1f} | {nw} | {wb:.4e} | {wb / a.cells:.1f} |",
@@ -124,15 +131,15 @@ def main(argv=None):
     if a.valu:
         cnt = {c: counter_means(a.valu, c) for c in ("SQ_WAVES", "SQ_INSTS_VALU",
                                                      "SQ_ACTIVE_INST_VALU")}
-        waves = cnt["SQ_WAVES"][_pick(cnt["SQ_WAVES"], a.kernel)][0]
-        insts = cnt["SQ_INSTS_VALU"][_pick(cnt["SQ_INSTS_VALU"], a.kernel)][0]
-        active = cnt["SQ_ACTIVE_INST_VALU"][_pick(cnt["SQ_ACTIVE_INST_VALU"], a.kernel)][0]
+        waves = cnt["SQ_WAVES"][_pick(cnt["SQ_WAVES"], a.kernel)][0] * per
+        insts = cnt["SQ_INSTS_VALU"][_pick(cnt["SQ_INSTS_VALU"], a.kernel)][0] * per
+        active = cnt["SQ_ACTIVE_INST_VALU"][_pick(cnt["SQ_ACTIVE_INST_VALU"], a.kernel)][0] * per
         # a wave64 VALU instruction occupies its SIMD for 4 cycles
         avail = a.simds * a.clock_ghz * 1e9 * avg_us * 1e-6
         issue = 4 * insts / avail
         flop_frac = a.flops * a.cells / (avg_us * 1e-6) / (a.peak_tflops * 1e12)
         lines += ["## VALU", "", "Command:", "", f"    {a.cmd_valu}", "",
-                  "| | per launch | per wave |", "|---|---|---|",
+                  "| | per step | per wave |", "|---|---|---|",
                   f"| SQ_WAVES | {waves:.4g} | |",
                   f"| SQ_INSTS_VALU | {insts:.4g} | {insts / waves:.0f} |",
                   f"| SQ_ACTIVE_INST_VALU | {active:.4g} | {active / waves:.0f} |", "",
