@@ -14,8 +14,10 @@
 //    in the arena's 64-cell tile layout, and for the mixed form the compact
 //    weights (per-tile FP16-exact reference, per-cell FP16 difference) as
 //    slab.cpp builds them.
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/cardioid_hip.h"
@@ -233,6 +235,46 @@ void device_slab_weights(hipStream_t stream, SlabSetupScratch &sc, const BoxGeom
    CDH_LAUNCH_CHECK();
 }
 
+// Scratch of cdh_fibre_conductivity_int_noalloc, one copy per device for the
+// life of the process: the sine/cosine table of the integer angles
+// -CDH_FIBRE_ANGLE_MAX..CDH_FIBRE_ANGLE_MAX (written once, at the first call
+// on the device) and, after it, the out-of-range counter (cleared by every
+// call).  16,400 B of table plus one 256 B slot for the counter.
+constexpr int kAngleTable = 2 * (2 * CDH_FIBRE_ANGLE_MAX + 1);
+constexpr int kScratchDoubles = kAngleTable + 32;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+
+__global__ void k_clear_u32(unsigned *x) { *x = 0u; }
+
+// The device address of this device's scratch, taken once per device (with
+// the device current) and the angle table written into it then.  Host code
+// passes this address, never the array's name, to kernels and copies.
+double *noalloc_scratch(hipStream_t s)
+{
+   static std::mutex mu;
+   static double *addr[kMaxDevices] = {};
+   int dev = -1;
+   CDH_HIP(hipGetDevice(&dev));
+   CDH_REQUIRE(dev >= 0 && dev < kMaxDevices, "noalloc scratch: device id out of range");
+   std::lock_guard<std::mutex> lock(mu);
+   if (!addr[dev]) {
+      double *p = nullptr;
+      CDH_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+      CDH_REQUIRE(p != nullptr, "noalloc scratch: no device address");
+      static double tab[kAngleTable];
+      for (int a = -CDH_FIBRE_ANGLE_MAX; a <= CDH_FIBRE_ANGLE_MAX; a++) {
+         const double ang = a * (M_PI / 256);
+         tab[2 * (a + CDH_FIBRE_ANGLE_MAX)] = std::sin(ang);
+         tab[2 * (a + CDH_FIBRE_ANGLE_MAX) + 1] = std::cos(ang);
+      }
+      CDH_HIP(hipMemcpyAsync(p, tab, sizeof(tab), hipMemcpyHostToDevice, s));
+      CDH_HIP(hipStreamSynchronize(s));
+      addr[dev] = p;
+   }
+   return addr[dev];
+}
+
 void device_fgr_weights(hipStream_t stream, const uint8_t *tissue, const double *sigma,
                         int64_t ncell, const int64_t *bidx, const fgr::WeightGeom &G, double *A0,
                         float *Aw, int64_t stride, bool blocked)
@@ -289,6 +331,34 @@ int cdh_fibre_conductivity_int(int64_t n, const int *theta, const int *phi, doub
       CDH_HIP(hipMemcpyAsync(&nbad, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
       CDH_HIP(hipStreamSynchronize(s));
       CDH_REQUIRE(nbad == 0, "cdh_fibre_conductivity_int: an angle outside [-512, 512]");
+   });
+}
+
+// The same without allocating: the table and the counter live in the
+// library's per-device scratch.  Calls that overlap on one device race on
+// the counter.
+int cdh_fibre_conductivity_int_noalloc(int64_t n, const int *theta, const int *phi,
+                                       double sigmaL, double sigmaT, double *sigma, void *stream)
+{
+   return guard([&] {
+      CDH_REQUIRE(n >= 0 && (n == 0 || (theta && phi && sigma)),
+                  "cdh_fibre_conductivity_int_noalloc: null argument");
+      CDH_REQUIRE(n <= INT32_MAX,
+                  "cdh_fibre_conductivity_int_noalloc: more than 2^31 - 1 cells in one call");
+      if (n == 0) return;
+      const hipStream_t s = (hipStream_t)stream;
+      double *tab = noalloc_scratch(s);
+      unsigned *bad = reinterpret_cast<unsigned *>(tab + kAngleTable);
+      hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(1), 0, s, bad);
+      CDH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_fibre_sigma_int, dim3(grid_for(n)), dim3(kBlock), 0, s, n, theta, phi,
+                         -CDH_FIBRE_ANGLE_MAX, CDH_FIBRE_ANGLE_MAX, (const double *)tab, sigmaL,
+                         sigmaT, sigma, bad);
+      CDH_LAUNCH_CHECK();
+      unsigned nbad = 0;
+      CDH_HIP(hipMemcpyAsync(&nbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+      CDH_HIP(hipStreamSynchronize(s));
+      CDH_REQUIRE(nbad == 0, "cdh_fibre_conductivity_int_noalloc: an angle outside [-512, 512]");
    });
 }
 

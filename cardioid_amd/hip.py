@@ -112,7 +112,7 @@ EXPORTS = [
     "cdh_sim_read_checkpoint", "cdh_fibre_conductivity", "cdh_fibre_conductivity_int",
     "cdh_fgr_weights", "cdh_batch_link_halo", "cdh_batch_write_checkpoint",
     "cdh_batch_read_checkpoint", "cdh_batch_get_sensors", "cdh_batch_set_sensors",
-    "cdh_stream_rate",
+    "cdh_stream_rate", "cdh_fibre_conductivity_int_noalloc",
 ]
 
 ERR_FULL = 5
@@ -219,6 +219,9 @@ def lib():
                                              C.c_double, C.c_void_p, C.c_void_p]
         L.cdh_fibre_conductivity_int.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_double,
                                                  C.c_double, C.c_void_p, C.c_void_p]
+        L.cdh_fibre_conductivity_int_noalloc.argtypes = [C.c_int64, C.c_void_p, C.c_void_p,
+                                                         C.c_double, C.c_double, C.c_void_p,
+                                                         C.c_void_p]
         L.cdh_fgr_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                       C.c_double, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]

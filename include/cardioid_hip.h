@@ -170,6 +170,16 @@ int cdh_fibre_conductivity(int64_t n, const double *theta, const double *phi, do
 #define CDH_FIBRE_ANGLE_MAX 512
 int cdh_fibre_conductivity_int(int64_t n, const int *theta, const int *phi, double sigmaL,
                                double sigmaT, double *sigma, void *stream);
+/* cdh_fibre_conductivity_int without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (a serving
+ * loop admitting requests, where hipFree would stall the device).  The
+ * angle table (16,400 B) and the out-of-range counter live in the library's
+ * own per-device scratch, which holds nothing else and is independent of
+ * n; the table is written at the first call on each device.  Calls that
+ * overlap on one device race on the counter.  Same arguments, results
+ * (bit for bit) and errors as cdh_fibre_conductivity_int; n <= 2^31 - 1. */
+int cdh_fibre_conductivity_int_noalloc(int64_t n, const int *theta, const int *phi,
+                                       double sigmaL, double sigmaT, double *sigma, void *stream);
 /* FGRDiffusionOMP::precomputeCoefficients (elec/FGRDiffusionOMP.cc:166-224)
  * on the device: for ncell cells at box points blockIndex[c] of a
  * bx*by*bz box (z fastest) holding tissue flags and sigma[6*], A0[c]
