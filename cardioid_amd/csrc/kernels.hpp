@@ -30,8 +30,10 @@ struct ReactionArgs {
    const double *Vm;
    double *dVm;
    double dt;
-   uint32_t gate_mask;          // bits 0..12 set; see gates64 and k_tt06_calc_f64
+   uint32_t gate_mask;          // bits 0..12 set; see gates64 and k_tt06_calc_pipe_f64
    const TT06ExactConst *E;     // configuration A (exact functions) constants, or null
+   double *sink;                // 20 x 64 doubles: the pipelined kernel's stores of
+                                // lanes past n (k_tt06_calc_pipe_f64)
 };
 
 struct StencilArgs {
@@ -91,18 +93,22 @@ __device__ __forceinline__ void apd_eval(const SensorArgs &a, int i, double v, d
 }
 
 // Stateful form, the reference's own bookkeeping.
-__device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v, double vrest)
+// The stateful form with the cell's two flags already loaded (activated,
+// active); only a change of state writes.
+__device__ __forceinline__ void sensor_eval_flags(const SensorArgs &a, int i, double v,
+                                                  double vrest, uint8_t activated, uint8_t active)
 {
-   if (!a.activated[i] && v > 0) {
+   if (!activated && v > 0) {
       a.activated[i] = 1;
       a.tact0[i] = a.time;
    }
-   if (a.active[i] && v < a.threshold) {
+   if (active && v < a.threshold) {
+      active = 0;
       a.active[i] = 0;
       if (a.trec[i] < 0 && a.tact[i] >= 0) a.trec[i] = a.time;
    }
    bool now = false;
-   if (!a.active[i] && v > a.threshold) {
+   if (!active && v > a.threshold) {
       a.active[i] = 1;
       if (a.tact[i] < 0) {
          a.tact[i] = a.time;
@@ -110,6 +116,11 @@ __device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v
       }
    }
    if (a.apd) apd_eval(a, i, v, vrest, now);
+}
+
+__device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v, double vrest)
+{
+   sensor_eval_flags(a, i, v, vrest, a.activated[i], a.active[i]);
 }
 
 // Pair form from (Vm before, Vm after) of the step: the state before the
@@ -158,17 +169,19 @@ __device__ __forceinline__ void range_report(RangeRec *r, uint64_t mask, int lan
 }
 
 // kernels (kernels_f64.hip)
-__global__ void k_tt06_calc_f64(ReactionArgs a);
+__global__ void k_tt06_calc_pipe_f64(ReactionArgs a);
 __global__ void k_tt06_gate_f64(ReactionArgs a);
 __global__ void k_tt06_calc_exact_f64(ReactionArgs a);
 __global__ void k_tt06_gate_exact_f64(ReactionArgs a);
 __global__ void k_scatter_f64(int n, const uint32_t *bi, const double *src, double *box);
+__global__ void k_scatter4_f64(int n, const uint32_t *bi, const double *src, double *box);
 __global__ void k_fgr_calc_f64(StencilArgs a);
 __global__ void k_integrate_f64(int n, double dt, double *Vm, const double *dVmR,
                                 const double *dVmD, const double *iStim);
 __global__ void k_fill_f64(int n, double v, double *x);
 __global__ void k_set_list_f64(int n, const int *list, double v, double *x);
 __global__ void k_sensor_f64(SensorArgs a, const double *Vm, const double *vrest);
+__global__ void k_sensor4_f64(SensorArgs a, const double *Vm, const double *vrest);
 __global__ void k_check_range_f64(int n, const double *Vm, const double *dVmR, const double *dVmD,
                                   double vmin, double vmax, int loop, RangeRec *r);
 

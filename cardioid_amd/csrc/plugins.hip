@@ -40,6 +40,8 @@ struct TT06Reaction {
    DevBuf<uint8_t> flags;
    DevBuf<double> state;
    DevBuf<uint8_t> flagsAllFixed;   // gates skipped (updateNonGate-only launches)
+   int64_t pipe_blocks = 0;         // resident workgroups of k_tt06_calc_pipe_f64
+   DevBuf<double> sink;             // its stores of lanes past n (20 x 64 doubles)
 };
 
 }  // namespace cdh
@@ -89,7 +91,39 @@ ReactionArgs reaction_args(cdh_tt06 *r, double dt, const int *index, int n, cons
    a.dt = dt;
    a.gate_mask = 0x1FFFu;
    a.E = r->exact ? r->Ed.p : nullptr;
+   a.sink = r->sink.p;
    return a;
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+// box[bi[c]] = src[c] for n cells: four cells per thread where both arrays
+// allow 16-byte loads
+void scatter(int64_t n, const uint32_t *bi, const double *src, double *box, hipStream_t s)
+{
+   if (aligned16(bi) && aligned16(src))
+      hipLaunchKernelGGL(k_scatter4_f64, dim3(grid_for((n + 3) / 4, kBlock)), dim3(kBlock), 0, s,
+                         (int)n, bi, src, box);
+   else
+      hipLaunchKernelGGL(k_scatter_f64, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, (int)n, bi,
+                         src, box);
+   CDH_LAUNCH_CHECK();
+}
+
+// The reaction over the object's n cells: configuration A's kernel, or the
+// pipelined kernel of configuration B on its resident grid.
+void launch_calc(cdh_tt06 *r, const ReactionArgs &a, hipStream_t s)
+{
+   if (r->exact) {
+      hipLaunchKernelGGL(k_tt06_calc_exact_f64, dim3(grid_for(a.n, kBlock)), dim3(kBlock), 0, s,
+                         a);
+   } else {
+      const int64_t tiles = ((int64_t)a.n + kBlk - 1) / kBlk;
+      const int64_t blocks = std::min<int64_t>((tiles + kBlock / 64 - 1) / (kBlock / 64),
+                                               r->pipe_blocks);
+      hipLaunchKernelGGL(k_tt06_calc_pipe_f64, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+   }
+   CDH_LAUNCH_CHECK();
 }
 
 const char *kFieldName[N_STATE] = {"Ca_i", "dVK_i", "Na_i", "Ca_ss", "Ca_SR", "R_prime",
@@ -170,6 +204,17 @@ int cdh_tt06_create(const cdh_tt06_params *p, int numPoints, cdh_tt06 **out)
          for (int64_t i = 0; i < r->stride; i++) tiled[blk_index(s, i, N_STATE)] = st[s * r->stride + i];
       r->state.alloc(tiled.size());
       r->state.upload(tiled.data(), tiled.size());
+      if (!exact) {
+         // the pipelined kernel's grid: as many workgroups as are resident at once
+         int dev = 0, ncu = 0, per_cu = 0;
+         CDH_HIP(hipGetDevice(&dev));
+         CDH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+         CDH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tt06_calc_pipe_f64,
+                                                              kBlock, 0));
+         CDH_REQUIRE(ncu > 0 && per_cu > 0, "cdh_tt06_create: no resident workgroups");
+         r->pipe_blocks = (int64_t)ncu * per_cu;
+         r->sink.alloc((N_STATE + 1) * kBlk);
+      }
       *out = r.release();
    });
 }
@@ -188,10 +233,7 @@ int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const do
       CDH_REQUIRE(r && Vm && dVm, "cdh_tt06_calc: null argument");
       CDH_REQUIRE(n == r->n, "cdh_tt06_calc: n differs from numPoints");
       if (n == 0) return;
-      ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVm, r->flags.p);
-      hipLaunchKernelGGL(r->exact ? k_tt06_calc_exact_f64 : k_tt06_calc_f64,
-                         dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
-      CDH_LAUNCH_CHECK();
+      launch_calc(r, reaction_args(r, dt, indexArray, n, Vm, dVm, r->flags.p), S(stream));
    });
 }
 
@@ -202,10 +244,8 @@ int cdh_tt06_update_nongate(cdh_tt06 *r, double dt, const int *indexArray, int n
       CDH_REQUIRE(r && Vm && dVR && n == r->n, "cdh_tt06_update_nongate: bad arguments");
       if (n == 0) return;
       // the calc kernel with every gate held fixed is the non-gate update
-      ReactionArgs a = reaction_args(r, dt, indexArray, n, Vm, dVR, r->flagsAllFixed.p);
-      hipLaunchKernelGGL(r->exact ? k_tt06_calc_exact_f64 : k_tt06_calc_f64,
-                         dim3(grid_for(n, kBlock)), dim3(kBlock), 0, S(stream), a);
-      CDH_LAUNCH_CHECK();
+      launch_calc(r, reaction_args(r, dt, indexArray, n, Vm, dVR, r->flagsAllFixed.p),
+                  S(stream));
    });
 }
 
@@ -393,9 +433,7 @@ int cdh_fgr_update_local_voltage(cdh_fgr *d, const double *VmLocal, void *stream
    return guard([&] {
       CDH_REQUIRE(d && VmLocal, "cdh_fgr_update_local_voltage: null argument");
       if (d->nLocal == 0) return;
-      hipLaunchKernelGGL(k_scatter_f64, dim3(grid_for(d->nLocal, kBlock)), dim3(kBlock), 0,
-                         S(stream), (int)d->nLocal, d->bi.p, VmLocal, d->box.p);
-      CDH_LAUNCH_CHECK();
+      scatter(d->nLocal, d->bi.p, VmLocal, d->box.p, S(stream));
    });
 }
 
@@ -405,9 +443,7 @@ int cdh_fgr_update_remote_voltage(cdh_fgr *d, const double *VmRemote, void *stre
       CDH_REQUIRE(d, "cdh_fgr_update_remote_voltage: null argument");
       if (d->nRemote == 0) return;
       CDH_REQUIRE(VmRemote, "cdh_fgr_update_remote_voltage: null argument");
-      hipLaunchKernelGGL(k_scatter_f64, dim3(grid_for(d->nRemote, kBlock)), dim3(kBlock), 0,
-                         S(stream), (int)d->nRemote, d->bi.p + d->nLocal, VmRemote, d->box.p);
-      CDH_LAUNCH_CHECK();
+      scatter(d->nRemote, d->bi.p + d->nLocal, VmRemote, d->box.p, S(stream));
    });
 }
 
@@ -507,8 +543,12 @@ int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, voi
       a.apd_phase = s->apd_phase.p;
       a.vmax = s->vmax.p;
       a.t90 = s->t90.p;
-      hipLaunchKernelGGL(k_sensor_f64, dim3(grid_for(s->n, kBlock)), dim3(kBlock), 0, S(stream), a,
-                         Vm, (const double *)s->vrest.p);
+      if (aligned16(Vm))
+         hipLaunchKernelGGL(k_sensor4_f64, dim3(grid_for((s->n + 3) / 4, kBlock)), dim3(kBlock), 0,
+                            S(stream), a, Vm, (const double *)s->vrest.p);
+      else
+         hipLaunchKernelGGL(k_sensor_f64, dim3(grid_for(s->n, kBlock)), dim3(kBlock), 0,
+                            S(stream), a, Vm, (const double *)s->vrest.p);
       CDH_LAUNCH_CHECK();
       s->evaluated = 1;
    });
