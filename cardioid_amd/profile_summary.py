@@ -134,21 +134,27 @@ def main(argv=None):
         waves = cnt["SQ_WAVES"][_pick(cnt["SQ_WAVES"], a.kernel)][0] * per
         insts = cnt["SQ_INSTS_VALU"][_pick(cnt["SQ_INSTS_VALU"], a.kernel)][0] * per
         active = cnt["SQ_ACTIVE_INST_VALU"][_pick(cnt["SQ_ACTIVE_INST_VALU"], a.kernel)][0] * per
+        tiles = a.cells / 64.0
         # a wave64 VALU instruction occupies its SIMD for 4 cycles
         avail = a.simds * a.clock_ghz * 1e9 * avg_us * 1e-6
         issue = 4 * insts / avail
         flop_frac = a.flops * a.cells / (avg_us * 1e-6) / (a.peak_tflops * 1e12)
         lines += ["## VALU", "", "Command:", "", f"    {a.cmd_valu}", "",
-                  "| | per step | per wave |", "|---|---|---|",
-                  f"| SQ_WAVES | {waves:.4g} | |",
-                  f"| SQ_INSTS_VALU | {insts:.4g} | {insts / waves:.0f} |",
-                  f"| SQ_ACTIVE_INST_VALU | {active:.4g} | {active / waves:.0f} |", "",
+                  "| | per step | per wave | per 64-cell tile |", "|---|---|---|---|",
+                  f"| SQ_WAVES | {waves:.4g} | | |",
+                  f"| SQ_INSTS_VALU | {insts:.4g} | {insts / waves:.0f} | "
+                  f"{insts / tiles:.0f} |",
+                  f"| SQ_ACTIVE_INST_VALU | {active:.4g} | {active / waves:.0f} | "
+                  f"{active / tiles:.0f} |", "",
+                  "(A wave of a kernel on the resident grid walks many tiles; the per-tile "
+                  "figure compares kernels of either kind.)", "",
                   f"VALU issue: 4 cycles x {insts:.4g} instructions over {a.simds} SIMDs x "
                   f"{a.clock_ghz} GHz x {avg_us:.1f} us = **{issue:.2f}** of the SIMDs' cycles "
                   f"(at the peak clock; a lower clock raises it).  Algorithmic arithmetic: "
                   f"{a.flops:g} flop per cell-step = {flop_frac * a.peak_tflops:.1f} TFLOP/s = "
                   f"{flop_frac:.3f} of {a.peak_tflops} TFLOP/s.", ""]
-        out.update(valu_instructions_per_wave=insts / waves, valu_issue_frac=issue,
+        out.update(valu_instructions_per_wave=insts / waves,
+                   valu_instructions_per_tile=insts / tiles, valu_issue_frac=issue,
                    flop_per_cell_step=a.flops, flop_frac=flop_frac,
                    peak_tflops=a.peak_tflops, clock_ghz=a.clock_ghz)
     with open(a.out + ".md", "w") as fh:
