@@ -343,6 +343,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--grid", type=str, default="256,256,512")
     ap.add_argument("--unfused-steps", type=int, default=500)
+    ap.add_argument("--exact-unfused-steps", type=int, default=300,
+                    help="the unfused loop in configuration A (TT06Dev tolerance 0, the "
+                         "factory's default deck) for this many steps")
     ap.add_argument("--mixed-steps", type=int, default=5000)
     ap.add_argument("--exact-steps", type=int, default=2000)
     ap.add_argument("--tune-steps", type=int, default=40)
@@ -398,6 +401,18 @@ def main():
         res["fp64_unfused"] = dict(ms=ms, steps=args.unfused_steps,
                                    rate=anat.ncell * args.unfused_steps / (ms * 1e-3) / 1e9)
         log(f"fp64_unfused: {res['fp64_unfused']['rate']:.3f} Gcell-steps/s")
+    if args.exact_unfused_steps > 0:
+        sim = hip.Simulation(anat, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims,
+                             mode="fp64_unfused", exact_functions=True, jhTauSmooth=1)
+        sim.run(args.warmup)
+        sim.sync()
+        ms = sim.run_timed(args.exact_unfused_steps)
+        sim.close()
+        res["fp64_unfused_exact"] = dict(
+            ms=ms, steps=args.exact_unfused_steps,
+            rate=anat.ncell * args.exact_unfused_steps / (ms * 1e-3) / 1e9)
+        log(f"fp64_unfused, configuration A: {res['fp64_unfused_exact']['rate']:.3f} "
+            f"Gcell-steps/s")
 
     f64, mx = res["fp64_fma"], res["mixed_fused"]
     ms_per_step = f64["ms"] / f64["steps"]
@@ -482,6 +497,14 @@ def main():
         out["fp64_unfused"] = {"value": u["rate"], "unit": "Gcell-steps/s", "steps": u["steps"],
                                "ms_per_step": u["ms"] / u["steps"],
                                "algorithmic_bytes_per_cell_step": UNFUSED_F64_BYTES}
+    if "fp64_unfused_exact" in res:
+        u = res["fp64_unfused_exact"]
+        out["fp64_unfused_configuration_A"] = {
+            "value": u["rate"], "unit": "Gcell-steps/s", "steps": u["steps"],
+            "ms_per_step": u["ms"] / u["steps"],
+            "note": "TT06Dev tolerance 0, mod 1 (the reaction factory's default deck): exact "
+                    "functions with the device's exp/log/pow, generic gate update; the same "
+                    "loop and diffusion as fp64_unfused"}
     if args.shard_steps > 0 and args.shard_parts > 1:
         sh = shards(hip, W, anat, stims, args.shard_parts, "fp64_fma", f64["launch"],
                     args.shard_steps, args.graph)
