@@ -145,3 +145,52 @@ def test_remote_only_neighbours_and_value_accessors():
     for cell, handle in ((0, 0), (4, 18), (2, 7)):
         assert st[handle * 5 + cell] == 0.125 + cell + handle
     L.cdh_tt06_destroy(h)
+
+
+def test_scatter_and_sensor_take_unaligned_arrays():
+    """The diffusion object's voltage scatter and the activation sensor read
+    four cells per thread with 16-byte loads when the caller's array allows
+    it, and one cell per thread when it does not (an array starting 8 bytes
+    into an allocation).  Both give the same bits, over a ragged cell count
+    (4199 = 4 * 1049 + 3)."""
+    L = hip.lib()
+    nx, ny, nz = 19, 17, 13
+    n = nx * ny * nz
+    say(f"scatter and sensor, aligned and unaligned arrays, {n} cells")
+    gid = np.arange(n, dtype=np.int64)
+    ct = np.ones(n, np.int32)
+    sig = np.tile([1.3e-4, 0.2e-4, 0.1e-4, 1.1e-4, 0.3e-4, 0.9e-4], n).astype(np.float64)
+    p = hip.FGRParams(nx=nx, ny=ny, nz=nz, dx=0.2, dy=0.2, dz=0.2, diffusionScale=714.0,
+                      nLocal=n, nRemote=0, gid=hip.ptr(gid, hip.i64p),
+                      cellType=ct.ctypes.data_as(hip.ip), sigma=hip.ptr(sig))
+    rng = np.random.default_rng(11)
+    steps = [rng.uniform(-90, 40, n) for _ in range(6)]
+    out = []
+    for shift in (0, 1):
+        d = C.c_void_p()
+        hip.check(L.cdh_fgr_create(C.byref(p), C.byref(d)), "cdh_fgr_create")
+        s = C.c_void_p()
+        hip.check(L.cdh_activation_sensor_create(n, -40.0, C.byref(s)), "sensor")
+        buf = torch.zeros(n + 2, dtype=torch.float64, device="cuda")
+        vm = buf[shift:shift + n]
+        assert (vm.data_ptr() % 16 == 0) == (shift == 0)
+        dvms = []
+        for k, v in enumerate(steps):
+            vm.copy_(torch.from_numpy(v))
+            dvm = torch.zeros(n, dtype=torch.float64, device="cuda")
+            hip.check(L.cdh_fgr_update_local_voltage(d, vp(vm), None), "update")
+            hip.check(L.cdh_fgr_calc(d, vp(dvm), None), "calc")
+            hip.check(L.cdh_activation_sensor_eval(s, 0.02 * (k + 1), vp(vm), None), "eval")
+            dvms.append(dvm.cpu().numpy())
+        torch.cuda.synchronize()
+        act = [np.zeros(n) for _ in range(3)]
+        hip.check(L.cdh_activation_sensor_get(s, *(hip.ptr(a) for a in act)), "get")
+        out.append((dvms, act))
+        L.cdh_activation_sensor_destroy(s)
+        L.cdh_fgr_destroy(d)
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+    # the sensors saw crossings: some times recorded, not all
+    assert 0 < np.count_nonzero(out[0][1][1] >= 0) < n
