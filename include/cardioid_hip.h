@@ -106,7 +106,8 @@ int cdh_tt06_destroy(cdh_tt06 *r);
 /* Reaction::calc (elec/Reaction.hh:15-17; TT06Dev :393-407): for i < n,
  * reads Vm[indexArray[i]], writes dVm[indexArray[i]] = -Iion, advances the
  * 19 states by dt.  indexArray may be NULL (identity).  iStim is ignored,
- * as TT06Dev ignores it. */
+ * as TT06Dev ignores it.  Vm and dVm are separate arrays (read-only and
+ * write-only in the reference's signature) and must not overlap. */
 int cdh_tt06_calc(cdh_tt06 *r, double dt, const int *indexArray, int n, const double *Vm,
                   const double *iStim, double *dVm, void *stream);
 /* Reaction::updateNonGate / updateGate (pdr loop; TT06Dev :408-453) */
