@@ -312,6 +312,12 @@ int cdh_batch_get_t90(cdh_batch *b, int slab, double *t90)
    });
 }
 
+int cdh_batch_sensor_counts(cdh_batch *b, int slab, int64_t *n_tact0, int64_t *n_tact,
+                            int64_t *n_trec)
+{
+   return guard([&] { engine(b).sensor_counts(slab, n_tact0, n_tact, n_trec); });
+}
+
 int cdh_batch_get_sensors(cdh_batch *b, int slab, double *rows, double *time, int *loop)
 {
    return guard([&] {

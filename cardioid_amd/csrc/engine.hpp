@@ -102,6 +102,8 @@ struct EngineArgs {
    double *tact0, *tact, *trec;
    uint8_t *apd_phase;
    double *vmax, *t90;
+   uint32_t *counts;            // per slab, 4 words: cells whose tact0, tact, trec have
+                                // been recorded (a wavefront ballot and one atomic each)
    RangeRec *ranges;            // per slab
 };
 
@@ -173,9 +175,10 @@ __device__ __forceinline__ void slab_advance(SlabDev &S, int parity, const SlabV
    S.stim_on[q] = on;
 }
 
-// Sensors at the time after the step, pair form (Vm before and after).
-__device__ __forceinline__ void engine_sensors(const EngineArgs &a, const SlabView &v, int64_t i,
-                                               double v0, double v1, double vrest)
+// Sensors at the time after the step, pair form (Vm before and after);
+// returns the cell's recorded times packed for sensor_count.
+__device__ __forceinline__ uint32_t engine_sensors(const EngineArgs &a, const SlabView &v,
+                                                   int64_t i, double v0, double v1, double vrest)
 {
    SensorArgs s;
    s.n = 0;
@@ -192,7 +195,8 @@ __device__ __forceinline__ void engine_sensors(const EngineArgs &a, const SlabVi
    s.apd_phase = a.apd_phase;
    s.vmax = a.vmax;
    s.t90 = a.t90;
-   sensor_eval_pair(s, (int)i, v0, v1, vrest);
+   s.counts = nullptr;
+   return sensor_events_packed(sensor_eval_pair(s, (int)i, v0, v1, vrest));
 }
 
 // Vm range check of a wavefront (cells whose Vm at the step's start is out

@@ -79,10 +79,13 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 #pragma unroll
       for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
    }
+   uint32_t ev = 0;
    if (live) {
       a.box_next[b] = Vn;
-      if (a.sensors) engine_sensors(a, v, i, V, Vn, P.Vm0);
+      if (a.sensors) ev = engine_sensors(a, v, i, V, Vn, P.Vm0);
    }
+   // every lane of the tile is here (padding lanes with no events)
+   if (a.sensors & 1) sensor_count(a.counts + 4 * sl, ev);
 }
 
 }  // namespace

@@ -344,6 +344,8 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    slabdev_.zero();
    track(ranges_, cap_slabs);
    ranges_.zero();
+   track(counts_, 4 * (size_t)cap_slabs);
+   counts_.zero();
    if (mode != CDH_MODE_MIXED_FUSED) {
       track(state_, (size_t)N_STATE * S);
       track(Aw_, (size_t)18 * S);
@@ -489,6 +491,7 @@ int Engine::add(const SlabHost &h)
       put(box_[0].p + b0, h.box0);
       put(box_[1].p + b0, h.box0);
       reset_sensors(s);
+      put(counts_.p + 4 * id, std::vector<uint32_t>(4, 0u));
       SlabDev d{};
       d.time[parity_] = 0.0;
       d.loop[parity_] = 0;
@@ -600,6 +603,7 @@ EngineArgs Engine::args(int parity) const
    a.apd_phase = apd_phase_.p;
    a.vmax = vmax_.p;
    a.t90 = t90_.p;
+   a.counts = counts_.p;
    a.ranges = ranges_.p;
    return a;
 }
@@ -789,6 +793,7 @@ double Engine::autotune(int steps, std::vector<std::pair<LaunchCfg, double>> *ta
    add(t90_.p, sizeof(double) * S);
    add(slabdev_.p, sizeof(SlabDev) * slabs_.size());
    add(ranges_.p, sizeof(RangeRec) * slabs_.size());
+   add(counts_.p, sizeof(uint32_t) * 4 * slabs_.size());
    std::vector<std::unique_ptr<DevBuf<uint8_t>>> saved;
    for (auto &r : regions) {
       saved.emplace_back(new DevBuf<uint8_t>(r.second));
@@ -934,6 +939,18 @@ void Engine::activation(int id, double *tact0, double *tact, double *trec)
    if (trec) download(trec_.p + c0, trec, s.n);
 }
 
+void Engine::sensor_counts(int id, int64_t *n_tact0, int64_t *n_tact, int64_t *n_trec)
+{
+   check_slab(id);
+   CDH_REQUIRE(sensors_ & 1, "engine: created without activation sensors");
+   sync();
+   uint32_t c[4];
+   download(counts_.p + 4 * id, c, 4);
+   if (n_tact0) *n_tact0 = c[0];
+   if (n_tact) *n_tact = c[1];
+   if (n_trec) *n_trec = c[2];
+}
+
 // fresh sensors over the slab's tiles: no activation, no APD90 yet
 void Engine::reset_sensors(const Slab &s)
 {
@@ -972,6 +989,7 @@ void Engine::set_clock(int id, double time, int loop)
    check_slab(id);
    sync();
    reset_sensors(slabs_[id]);
+   put(counts_.p + 4 * id, std::vector<uint32_t>(4, 0u));
    write_clock(id, time, loop, loop);
 }
 
@@ -1010,6 +1028,11 @@ void Engine::set_sensor_state(int id, const double *rows, double time, int loop)
       put(tact0_.p + c0, std::vector<double>(rows, rows + n));
       put(tact_.p + c0, std::vector<double>(rows + n, rows + 2 * n));
       put(trec_.p + c0, std::vector<double>(rows + 2 * n, rows + 3 * n));
+      // the counts the step kernels keep, for the times as set
+      std::vector<uint32_t> c(4, 0u);
+      for (int q = 0; q < 3; q++)
+         for (int64_t k = 0; k < n; k++) c[q] += rows[q * n + k] >= 0.0;
+      put(counts_.p + 4 * id, c);
    }
    if (sensors_ & 2) {
       std::vector<uint8_t> ph(n);

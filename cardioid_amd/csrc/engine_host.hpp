@@ -109,6 +109,10 @@ class Engine {
    // (the step's Vm before is their previous one)
    void set_sensor_state(int id, const double *rows, double time, int loop);
    void activation(int id, double *tact0, double *tact, double *trec);   // slab order
+   // cells of the slab whose tact0, tact and trec have been recorded, kept
+   // on the device by the step kernels (one wavefront ballot and atomic per
+   // kind that occurs); the sensors' own counts, reset with them
+   void sensor_counts(int id, int64_t *n_tact0, int64_t *n_tact, int64_t *n_trec);
    void t90(int id, double *out);
    // out-of-range record since the last call (cell as a slab-order index);
    // clears it
@@ -212,6 +216,7 @@ class Engine {
    DevBuf<double> tact0_, tact_, trec_, vmax_, t90_;
    DevBuf<uint8_t> apd_phase_;
    DevBuf<RangeRec> ranges_;
+   DevBuf<uint32_t> counts_;      // per slab: recorded tact0, tact, trec, (unused)
 };
 
 // mixed-mode FP32 slots (tt06_mixed.hpp order) and the reference slot each

@@ -118,10 +118,13 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    for (int s = 0; s < N_STATE32; s++) stream_store_f32(s32[s], R32, 4u * lc, 4u * rw * s);
 #pragma unroll
    for (int s = 0; s < N_STATE64; s++) stream_store_f64(s64[s], R64, 8u * lc, 8u * rw * s);
+   uint32_t ev = 0;
    if (live) {
       a.box_next[b] = Vn;
-      if (a.sensors) engine_sensors(a, v, i, V, Vn, P.Vm0);
+      if (a.sensors) ev = engine_sensors(a, v, i, V, Vn, P.Vm0);
    }
+   // every lane of the tile is here (padding lanes with no events)
+   if (a.sensors & 1) sensor_count(a.counts + 4 * sl, ev);
 }
 
 }  // namespace

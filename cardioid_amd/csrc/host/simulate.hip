@@ -51,6 +51,12 @@ void ActivationSensor::get(double *tact0, double *tact, double *trec) const
    check(cdh_activation_sensor_get(h_, tact0, tact, trec));
 }
 
+void ActivationSensor::counts(int64_t *n_tact0, int64_t *n_tact, int64_t *n_trec) const
+{
+   CDH_HIP(hipStreamSynchronize(stream_));
+   check(cdh_activation_sensor_counts(h_, n_tact0, n_tact, n_trec));
+}
+
 void ActivationSensor::t90(double *out) const
 {
    CDH_HIP(hipStreamSynchronize(stream_));

@@ -44,6 +44,7 @@ class ActivationSensor {
    ~ActivationSensor();
    void eval(double time, const double *Vm);
    void get(double *tact0, double *tact, double *trec) const;
+   void counts(int64_t *n_tact0, int64_t *n_tact, int64_t *n_trec) const;
    void t90(double *out) const;
 
  private:

@@ -70,7 +70,36 @@ struct SensorArgs {
    uint8_t *apd_phase;          // 0 waiting, 1 tracking Vmax, 2 done
    double *vmax;
    double *t90;
+   uint32_t *counts;            // cells whose tact0, tact, trec have been recorded
+                                // (3 counters; kept by the drop-in sensor kernels)
 };
+
+// Events of one sensor evaluation: which of the cell's times it recorded.
+constexpr unsigned kEvTact0 = 1u, kEvTact = 2u, kEvTrec = 4u;
+
+// Adds a wavefront's recorded times to counts[0..2] (tact0, tact, trec):
+// one ballot per kind, one atomic per kind that occurred, from the lowest
+// lane that took part.  Call from every lane that evaluated a cell (the
+// others contribute nothing; lanes can hold several cells' events summed in
+// `per_lane`, one count per kind, packed 8 bits each).
+__device__ __forceinline__ void sensor_count(uint32_t *counts, uint32_t per_lane)
+{
+   const uint64_t any = __ballot(per_lane != 0);
+   if (!any) return;
+   const int lane = threadIdx.x & 63;
+   for (int k = 0; k < 3; k++) {
+      uint32_t c = (per_lane >> (8 * k)) & 0xFFu;
+      for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+      if (c && lane == __builtin_ctzll(any)) atomicAdd(counts + k, c);
+   }
+}
+
+// per-lane packing of one cell's events for sensor_count
+__device__ __forceinline__ uint32_t sensor_events_packed(unsigned ev)
+{
+   return ((ev & kEvTact0) ? 1u : 0u) | ((ev & kEvTact) ? 1u << 8 : 0u) |
+          ((ev & kEvTrec) ? 1u << 16 : 0u);
+}
 
 __device__ __forceinline__ void apd_eval(const SensorArgs &a, int i, double v, double vrest,
                                          bool activated_now)
@@ -95,17 +124,24 @@ __device__ __forceinline__ void apd_eval(const SensorArgs &a, int i, double v, d
 // Stateful form, the reference's own bookkeeping.
 // The stateful form with the cell's two flags already loaded (activated,
 // active); only a change of state writes.
-__device__ __forceinline__ void sensor_eval_flags(const SensorArgs &a, int i, double v,
-                                                  double vrest, uint8_t activated, uint8_t active)
+// Returns the times it recorded (kEv* bits).
+__device__ __forceinline__ unsigned sensor_eval_flags(const SensorArgs &a, int i, double v,
+                                                      double vrest, uint8_t activated,
+                                                      uint8_t active)
 {
+   unsigned ev = 0;
    if (!activated && v > 0) {
       a.activated[i] = 1;
       a.tact0[i] = a.time;
+      ev |= kEvTact0;
    }
    if (active && v < a.threshold) {
       active = 0;
       a.active[i] = 0;
-      if (a.trec[i] < 0 && a.tact[i] >= 0) a.trec[i] = a.time;
+      if (a.trec[i] < 0 && a.tact[i] >= 0) {
+         a.trec[i] = a.time;
+         ev |= kEvTrec;
+      }
    }
    bool now = false;
    if (!active && v > a.threshold) {
@@ -113,32 +149,45 @@ __device__ __forceinline__ void sensor_eval_flags(const SensorArgs &a, int i, do
       if (a.tact[i] < 0) {
          a.tact[i] = a.time;
          now = true;
+         ev |= kEvTact;
       }
    }
    if (a.apd) apd_eval(a, i, v, vrest, now);
+   return ev;
 }
 
-__device__ __forceinline__ void sensor_eval(const SensorArgs &a, int i, double v, double vrest)
+__device__ __forceinline__ unsigned sensor_eval(const SensorArgs &a, int i, double v,
+                                                double vrest)
 {
-   sensor_eval_flags(a, i, v, vrest, a.activated[i], a.active[i]);
+   return sensor_eval_flags(a, i, v, vrest, a.activated[i], a.active[i]);
 }
 
 // Pair form from (Vm before, Vm after) of the step: the state before the
 // step is "Vm above the level" (false before the first evaluation), which
 // equals the stateful form unless Vm lands exactly on the level.
-__device__ __forceinline__ void sensor_eval_pair(const SensorArgs &a, int i, double v0, double v1,
-                                                 double vrest)
+// Returns the times it recorded (kEv* bits).
+__device__ __forceinline__ unsigned sensor_eval_pair(const SensorArgs &a, int i, double v0,
+                                                     double v1, double vrest)
 {
+   unsigned ev = 0;
    const bool was0 = !a.first_eval && v0 > 0;
-   if (!was0 && v1 > 0 && a.tact0[i] < 0) a.tact0[i] = a.time;
+   if (!was0 && v1 > 0 && a.tact0[i] < 0) {
+      a.tact0[i] = a.time;
+      ev |= kEvTact0;
+   }
    const bool wasT = !a.first_eval && v0 > a.threshold;
    bool now = false;
    if (!wasT && v1 > a.threshold && a.tact[i] < 0) {
       a.tact[i] = a.time;
       now = true;
+      ev |= kEvTact;
    }
-   if (wasT && v1 < a.threshold && a.trec[i] < 0 && a.tact[i] >= 0) a.trec[i] = a.time;
+   if (wasT && v1 < a.threshold && a.trec[i] < 0 && a.tact[i] >= 0) {
+      a.trec[i] = a.time;
+      ev |= kEvTrec;
+   }
    if (a.apd) apd_eval(a, i, v1, vrest, now);
+   return ev;
 }
 
 // Voltage range check (Simulate::checkRanges, elec/Simulate.cc:35-53):

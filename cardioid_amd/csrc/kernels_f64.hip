@@ -260,32 +260,13 @@ __global__ __launch_bounds__(256) void k_set_list_f64(int n, const int *__restri
    if (i < n) x[list[i]] = v;
 }
 
-// Integrator fused with the activation sensors (pair form: Vm before and
-// after the step are both at hand here).
-__global__ __launch_bounds__(256) void k_integrate_sensor_f64(int n, double dt,
-                                                              double *__restrict__ Vm,
-                                                              const double *__restrict__ dVmR,
-                                                              const double *__restrict__ dVmD,
-                                                              const double *__restrict__ iStim,
-                                                              SensorArgs s,
-                                                              const ParamSet *__restrict__ params,
-                                                              const uint8_t *__restrict__ pid)
-{
-   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i >= n) return;
-   const double v0 = Vm[i];
-   const double v1 = v0 + dt * (dVmR[i] + dVmD[i] + iStim[i]);
-   Vm[i] = v1;
-   if (s.enabled) sensor_eval_pair(s, i, v0, v1, s.apd ? params[pid[i]].Vm0 : 0.0);
-}
-
 // Sensor evaluation at (time, loop) after the step; Vm in cell order.
 __global__ __launch_bounds__(256) void k_sensor_f64(SensorArgs a, const double *__restrict__ Vm,
                                                     const double *__restrict__ vrest)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i >= a.n) return;
-   sensor_eval(a, i, Vm[i], a.apd ? vrest[i] : 0.0);
+   const uint32_t ev = i < a.n ? sensor_events_packed(sensor_eval(a, i, Vm[i], a.apd ? vrest[i] : 0.0)) : 0u;
+   sensor_count(a.counts, ev);
 }
 
 // The same, four cells per thread: the flags of the four as one 4-byte
@@ -294,6 +275,7 @@ __global__ __launch_bounds__(256) void k_sensor4_f64(SensorArgs a, const double 
                                                      const double *__restrict__ vrest)
 {
    const int64_t i = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+   uint32_t ev = 0;
    if (i + 3 < a.n) {
       const uint32_t acd = *reinterpret_cast<const uint32_t *>(a.activated + i);
       const uint32_t act = *reinterpret_cast<const uint32_t *>(a.active + i);
@@ -302,11 +284,15 @@ __global__ __launch_bounds__(256) void k_sensor4_f64(SensorArgs a, const double 
       const double v[4] = {v01.x, v01.y, v23.x, v23.y};
 #pragma unroll
       for (int k = 0; k < 4; k++)
-         sensor_eval_flags(a, (int)i + k, v[k], a.apd ? vrest[i + k] : 0.0,
-                           (uint8_t)(acd >> (8 * k)), (uint8_t)(act >> (8 * k)));
+         ev += sensor_events_packed(sensor_eval_flags(a, (int)i + k, v[k],
+                                                      a.apd ? vrest[i + k] : 0.0,
+                                                      (uint8_t)(acd >> (8 * k)),
+                                                      (uint8_t)(act >> (8 * k))));
    } else {
-      for (int k = (int)i; k < a.n; k++) sensor_eval(a, k, Vm[k], a.apd ? vrest[k] : 0.0);
+      for (int k = (int)i; k < a.n; k++)
+         ev += sensor_events_packed(sensor_eval(a, k, Vm[k], a.apd ? vrest[k] : 0.0));
    }
+   sensor_count(a.counts, ev);
 }
 
 // Simulate::checkRanges on Vm before the integrator (elec/simulationLoop.cc:214-217).

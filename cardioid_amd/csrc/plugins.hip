@@ -68,6 +68,7 @@ struct cdh_sensor {
    int apd = 0;
    DevBuf<uint8_t> apd_phase;
    DevBuf<double> vmax, t90, vrest;
+   DevBuf<uint32_t> counts;   // recorded tact0, tact, trec (and one unused word)
 };
 
 namespace {
@@ -519,6 +520,8 @@ int cdh_activation_sensor_create(int64_t n, double threshold, cdh_sensor **out)
       s->active.zero();
       s->activated.alloc(m);
       s->activated.zero();
+      s->counts.alloc(4);
+      s->counts.zero();
       *out = s.release();
    });
 }
@@ -543,6 +546,7 @@ int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, voi
       a.apd_phase = s->apd_phase.p;
       a.vmax = s->vmax.p;
       a.t90 = s->t90.p;
+      a.counts = s->counts.p;
       if (aligned16(Vm))
          hipLaunchKernelGGL(k_sensor4_f64, dim3(grid_for((s->n + 3) / 4, kBlock)), dim3(kBlock), 0,
                             S(stream), a, Vm, (const double *)s->vrest.p);
@@ -551,6 +555,20 @@ int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, voi
                             S(stream), a, Vm, (const double *)s->vrest.p);
       CDH_LAUNCH_CHECK();
       s->evaluated = 1;
+   });
+}
+
+int cdh_activation_sensor_counts(cdh_sensor *s, int64_t *n_tact0, int64_t *n_tact,
+                                  int64_t *n_trec)
+{
+   return guard([&] {
+      CDH_REQUIRE(s, "cdh_activation_sensor_counts: null argument");
+      uint32_t c[4] = {0, 0, 0, 0};
+      CDH_HIP(hipDeviceSynchronize());
+      if (s->n > 0) s->counts.download(c, 4);
+      if (n_tact0) *n_tact0 = c[0];
+      if (n_tact) *n_tact = c[1];
+      if (n_trec) *n_trec = c[2];
    });
 }
 

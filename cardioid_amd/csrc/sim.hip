@@ -439,6 +439,26 @@ int cdh_sim_get_activation(cdh_sim *s, double *tact0, double *tact, double *trec
    });
 }
 
+int cdh_sim_sensor_counts(cdh_sim *s, int64_t *n_tact0, int64_t *n_tact, int64_t *n_trec)
+{
+   return guard([&] {
+      CDH_REQUIRE(s, "cdh_sim_sensor_counts: null");
+      CDH_REQUIRE(s->p.sensors & 1, "cdh_sim_sensor_counts: the engine was created without sensors");
+      if (s->n == 0) {
+         if (n_tact0) *n_tact0 = 0;
+         if (n_tact) *n_tact = 0;
+         if (n_trec) *n_trec = 0;
+         return;
+      }
+      sim_sync(s);
+      if (s->loopsim) {
+         s->loopsim->sensor[0]->counts(n_tact0, n_tact, n_trec);
+         return;
+      }
+      s->eng->sensor_counts(s->slab, n_tact0, n_tact, n_trec);
+   });
+}
+
 int cdh_sim_get_t90(cdh_sim *s, double *t90)
 {
    return guard([&] {

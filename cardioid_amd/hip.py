@@ -113,6 +113,7 @@ EXPORTS = [
     "cdh_fgr_weights", "cdh_batch_link_halo", "cdh_batch_write_checkpoint",
     "cdh_batch_read_checkpoint", "cdh_batch_get_sensors", "cdh_batch_set_sensors",
     "cdh_stream_rate", "cdh_fibre_conductivity_int_noalloc",
+    "cdh_activation_sensor_counts", "cdh_sim_sensor_counts", "cdh_batch_sensor_counts",
 ]
 
 ERR_FULL = 5
@@ -168,6 +169,9 @@ def lib():
         L.cdh_activation_sensor_create.argtypes = [C.c_int64, C.c_double, C.POINTER(C.c_void_p)]
         L.cdh_activation_sensor_eval.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
         L.cdh_activation_sensor_get.argtypes = [C.c_void_p, dp, dp, dp]
+        L.cdh_activation_sensor_counts.argtypes = [C.c_void_p, i64p, i64p, i64p]
+        L.cdh_sim_sensor_counts.argtypes = [C.c_void_p, i64p, i64p, i64p]
+        L.cdh_batch_sensor_counts.argtypes = [C.c_void_p, C.c_int, i64p, i64p, i64p]
         L.cdh_activation_sensor_destroy.argtypes = [C.c_void_p]
         L.cdh_sim_create.argtypes = [C.POINTER(SimParams), C.c_int, C.POINTER(BoxStim),
                                      C.POINTER(C.c_void_p)]
@@ -400,6 +404,13 @@ class Simulation:
         check(lib().cdh_sim_get_t90(self.h, ptr(out)), "cdh_sim_get_t90")
         return out
 
+    def sensor_counts(self):
+        """(cells with tact0, with tact, with trec recorded), kept on the device."""
+        c = [C.c_int64() for _ in range(3)]
+        check(lib().cdh_sim_sensor_counts(self.h, *(C.byref(x) for x in c)),
+              "cdh_sim_sensor_counts")
+        return tuple(x.value for x in c)
+
     def range_report(self):
         """Out-of-range Vm since the last call: None, or a dict with the
         count of (cell, step) pairs and the values Simulate::outOfRange
@@ -626,6 +637,13 @@ class Batch:
         out = np.zeros(self.ncell[sid])
         check(lib().cdh_batch_get_t90(self.h, sid, ptr(out)), "cdh_batch_get_t90")
         return out
+
+    def sensor_counts(self, sid: int):
+        """(cells with tact0, with tact, with trec recorded) in slab sid."""
+        c = [C.c_int64() for _ in range(3)]
+        check(lib().cdh_batch_sensor_counts(self.h, sid, *(C.byref(x) for x in c)),
+              "cdh_batch_sensor_counts")
+        return tuple(x.value for x in c)
 
     def range_report(self, sid: int):
         return _range_report(lambda *a: lib().cdh_batch_range_report(self.h, sid, *a),

@@ -206,6 +206,12 @@ typedef struct cdh_sensor cdh_sensor;
 int cdh_activation_sensor_create(int64_t n, double threshold, cdh_sensor **out);
 int cdh_activation_sensor_eval(cdh_sensor *s, double time, const double *Vm, void *stream);
 int cdh_activation_sensor_get(cdh_sensor *s, double *tact0, double *tact, double *trec);
+/* How many cells have their tact0, tact and trec recorded, counted on the
+ * device as they are recorded (one wavefront ballot and atomic per kind):
+ * the "cells activated" figure without copying the arrays.  No reference
+ * counterpart (the reference's sensors keep only the per-cell arrays). */
+int cdh_activation_sensor_counts(cdh_sensor *s, int64_t *n_tact0, int64_t *n_tact,
+                                 int64_t *n_trec);
 int cdh_activation_sensor_destroy(cdh_sensor *s);
 /* APD90 on the same sensor (no counterpart in the reference): after the
  * first up-crossing of the threshold, t90 is the first evaluation time with
@@ -284,6 +290,8 @@ int cdh_sim_get_activation(cdh_sim *s, double *tact0, double *tact, double *trec
  * threshold, t90 is the first time Vm < Vrest + 0.1*(Vmax - Vrest), with
  * Vrest the cell type's rest Vm and Vmax the largest Vm since activation. */
 int cdh_sim_get_t90(cdh_sim *s, double *t90);
+/* cdh_activation_sensor_counts for the simulation's sensors (any mode) */
+int cdh_sim_sensor_counts(cdh_sim *s, int64_t *n_tact0, int64_t *n_tact, int64_t *n_trec);
 /* Vm range check (Simulate::checkRanges, Simulate.cc:35-53, before the
  * integrator): the number of (cell, step) pairs whose Vm left [vm_min,
  * vm_max] since the last call, and one of them (the first step's) with the
@@ -430,6 +438,10 @@ int cdh_batch_get_vm(cdh_batch *b, int slab, double *vm);
 int cdh_batch_get_state(cdh_batch *b, int slab, double *state);      /* [19][ncell] */
 int cdh_batch_get_activation(cdh_batch *b, int slab, double *tact0, double *tact, double *trec);
 int cdh_batch_get_t90(cdh_batch *b, int slab, double *t90);
+/* cdh_activation_sensor_counts for one slab; reset with its sensors (add,
+ * restart), set from the times by cdh_batch_set_sensors */
+int cdh_batch_sensor_counts(cdh_batch *b, int slab, int64_t *n_tact0, int64_t *n_tact,
+                            int64_t *n_trec);
 int cdh_batch_range_report(cdh_batch *b, int slab, int64_t *count, int *loop, int64_t *cell,
                            double *V, double *dVmR, double *dVmD);
 int cdh_batch_slab_info(cdh_batch *b, int slab, int64_t *ncell, double *time, int *loop);
