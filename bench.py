@@ -52,14 +52,27 @@ def log(msg):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it, with family, model, stepping
+    and clock: a virtualised host may report only a short model name."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if info:
+                        break            # the first processor's block is enough
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                info.setdefault(k, v)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    extra = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+    if "cpu MHz" in info:
+        extra.append(f"{float(info['cpu MHz']):.0f} MHz")
+    if extra:
+        name += " (" + ", ".join(extra) + ")"
+    return name
 
 
 def host_threads():
