@@ -367,7 +367,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
-    ap.add_argument("--serve-cap", type=int, default=0,
+    ap.add_argument("--serve-cap", type=int, default=6_000_000,
                     help="also serve the stream with at most this many cells in flight")
     ap.add_argument("--steady-requests", type=int, default=100,
                     help="a second stream below capacity: this many requests ...")
