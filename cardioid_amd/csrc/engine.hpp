@@ -139,6 +139,9 @@ struct SlabView {
    uint32_t stim_on;
    double scale, threshold;
    int param_base;
+   int check_range;             // the range check's switch and limits, read here at the
+   double vm_min, vm_max;       // kernel's start: read after the state stores, they
+                                // became vector loads that waited for those stores
 };
 
 __device__ __forceinline__ void slab_view(const SlabDev &S, int parity, double dt, SlabView &v)
@@ -150,6 +153,9 @@ __device__ __forceinline__ void slab_view(const SlabDev &S, int parity, double d
    v.threshold = S.threshold;
    v.param_base = S.param_base;
    v.stim_on = S.stim_on[parity];
+   v.check_range = S.check_range;
+   v.vm_min = S.vm_min;
+   v.vm_max = S.vm_max;
 }
 
 // The cell's stimulus; later stimuli overwrite earlier ones, as BoxStimulus
@@ -201,12 +207,13 @@ __device__ __forceinline__ uint32_t engine_sensors(const EngineArgs &a, const Sl
 
 // Vm range check of a wavefront (cells whose Vm at the step's start is out
 // of range); uniform branch, one atomic per wavefront that finds any.
-__device__ __forceinline__ void engine_range(const EngineArgs &a, const SlabDev &S, int sl,
-                                             int parity, bool live, int lane, int64_t i, double V,
-                                             double dVR, double dVD)
+__device__ __forceinline__ void engine_range(const EngineArgs &a, const SlabDev &S,
+                                             const SlabView &v, int sl, int parity, bool live,
+                                             int lane, int64_t i, double V, double dVR,
+                                             double dVD)
 {
-   if (!S.check_range) return;
-   const bool out = live && (V > S.vm_max || V < S.vm_min);
+   if (!v.check_range) return;
+   const bool out = live && (V > v.vm_max || V < v.vm_min);
    const uint64_t m = __ballot(out);
    if (m) range_report(a.ranges + sl, m, lane, S.loop[parity], i, V, dVR, dVD);
 }

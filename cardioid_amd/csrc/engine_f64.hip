@@ -62,17 +62,17 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    double dVR = 0.0;
    if (a.gate_mask & (1u << 12)) dVR = nongate64<FMA>(*a.K, P, a.coef, a.dt, V, st);
    gates64<FMA>(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, st, a.gate_mask);
-   // The states are stored once the range check and the stimulus have read
-   // the slab's descriptor (reads after a store become vector loads that
-   // wait for it) in the reference-rounding instance; the FMA instance keeps
-   // its stores first.  Measured back to back on one MI355X (20.1 M-cell
-   // shell): 1.577 against 1.594 ms per step with reference rounding; the
-   // FMA instance ran 1.3 % slower with the later stores.
+   // Reads of the slab's descriptor after a vector store become vector loads
+   // that wait for it, so the range check's limits come from the view read
+   // at the start.  Since then the stores' place makes no difference: on one
+   // MI355X, both instances within 0.2 % with the stores before or after the
+   // range check and the stimulus (the FMA step 0.5 % faster than when the
+   // limits were read here).
    if constexpr (FMA) {
 #pragma unroll
       for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
    }
-   engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
+   engine_range(a, S, v, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);
    if constexpr (!FMA) {

@@ -109,7 +109,7 @@ __device__ __forceinline__ void cell_mixed(const EngineArgs &a, const SlabDev &S
    }
    const double dVD = (double)acc * (v.scale * (double)S.wdelta_scale);
 
-   engine_range(a, S, sl, a.parity, live, lane, i, V, dVR, dVD);
+   engine_range(a, S, v, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);
    // K_i follows dVK_i = K_i/c9 + Vm of the reference: dK_i = c9*dt*(iK - dVm_D - iStim)
