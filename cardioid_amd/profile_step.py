@@ -8,7 +8,9 @@ FETCH_SIZE / WRITE_SIZE.
 MODE is fp64_fused, fp64_fma or mixed_fused (the fused engine; WAVES/BLOCK/OCC
 the launch configuration: register target, workgroup size, occupancy cap), or
 fp64_unfused (the reference loop over the drop-in objects: reaction, scatter,
-FGR stencil, integrator and sensor kernels; WAVES/BLOCK/OCC ignored).  Meant
+FGR stencil, integrator and sensor kernels; WAVES/BLOCK/OCC ignored), or
+fp64_unfused_exact (the same loop with the reaction factory's default deck,
+TT06Dev tolerance 0: the exact functions).  Meant
 to run under rocprofv3 (kernel trace, or one --pmc counter per run).
 """
 from __future__ import annotations
@@ -32,10 +34,12 @@ def main(argv):
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
-    if mode == "fp64_unfused":
+    if mode in ("fp64_unfused", "fp64_unfused_exact"):
         anat, depth = W.ellipsoid_shell(256, 256, 512, seed=0)
         stims = W.endocardial_point_stimuli(anat, depth, 8, seed=0)
-        sim = hip.Simulation(anat, W.CELLTYPES_EMI, 0.02, W.DIFFUSION_SCALE, stims, mode=mode)
+        kw = dict(exact_functions=True, jhTauSmooth=1) if mode.endswith("exact") else {}
+        sim = hip.Simulation(anat, W.CELLTYPES_EMI, 0.02, W.DIFFUSION_SCALE, stims,
+                             mode="fp64_unfused", **kw)
         sim.run(steps)
         sim.sync()
         sim.close()

@@ -76,7 +76,8 @@ def main(argv=None):
                     "SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES GRBM_GUI_ACTIVE GRBM_COUNT")
     ap.add_argument("--cmd-valu", default="")
     ap.add_argument("--flops", type=float, default=1060.0,
-                    help="algorithmic floating-point operations per cell-step (FMA = 2)")
+                    help="algorithmic floating-point operations per cell-step (FMA = 2); "
+                         "0 leaves the flop fraction out")
     ap.add_argument("--peak-tflops", type=float, default=78.6)
     ap.add_argument("--simds", type=int, default=1024)
     ap.add_argument("--clock-ghz", type=float, default=2.4)
@@ -139,6 +140,9 @@ def main(argv=None):
         avail = a.simds * a.clock_ghz * 1e9 * avg_us * 1e-6
         issue = 4 * insts / avail
         flop_frac = a.flops * a.cells / (avg_us * 1e-6) / (a.peak_tflops * 1e12)
+        flop_line = (f"  Algorithmic arithmetic: {a.flops:g} flop per cell-step = "
+                     f"{flop_frac * a.peak_tflops:.1f} TFLOP/s = {flop_frac:.3f} of "
+                     f"{a.peak_tflops} TFLOP/s." if a.flops > 0 else "")
         lines += ["## VALU", "", "Command:", "", f"    {a.cmd_valu}", "",
                   "| | per step | per wave | per 64-cell tile |", "|---|---|---|---|",
                   f"| SQ_WAVES | {waves:.4g} | | |",
@@ -150,9 +154,7 @@ def main(argv=None):
                   "figure compares kernels of either kind.)", "",
                   f"VALU issue: 4 cycles x {insts:.4g} instructions over {a.simds} SIMDs x "
                   f"{a.clock_ghz} GHz x {avg_us:.1f} us = **{issue:.2f}** of the SIMDs' cycles "
-                  f"(at the peak clock; a lower clock raises it).  Algorithmic arithmetic: "
-                  f"{a.flops:g} flop per cell-step = {flop_frac * a.peak_tflops:.1f} TFLOP/s = "
-                  f"{flop_frac:.3f} of {a.peak_tflops} TFLOP/s.", ""]
+                  f"(at the peak clock; a lower clock raises it).{flop_line}", ""]
         out.update(valu_instructions_per_wave=insts / waves,
                    valu_instructions_per_tile=insts / tiles, valu_issue_frac=issue,
                    flop_per_cell_step=a.flops, flop_frac=flop_frac,
