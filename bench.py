@@ -191,14 +191,15 @@ def stream_ceiling(hip, cells, reps=20):
 
 
 def fused_run(hip, slab, ncell, mode, steps, warmup, tune_steps, graph, sensors=1,
-              prebuild=True):
+              prebuild=True, exact=False):
     """One slab in the engine: autotuned launch, graph replay, HIP-event time.
     The set-up (autotune, then one graph length of steps that captures the
     graph, plus one step when the warm-up count is odd) runs before the
     warm-up, so that the timed steps start on a graph boundary with the graph
     built: the timed region replays graphs and builds none."""
     cells, box, _ = slab.extent()
-    b = hip.Batch(DT, cells, box, slab_capacity=1, mode=mode, sensors=sensors)
+    b = hip.Batch(DT, cells, box, slab_capacity=1, mode=mode, sensors=sensors,
+                  exact_functions=exact, jhTauSmooth=1 if exact else 0)
     sid = b.add(slab)
     b.sync()
     best, table = b.autotune(tune_steps) if tune_steps > 0 else (None, [])
@@ -359,6 +360,8 @@ def main():
     ap.add_argument("--exact-unfused-steps", type=int, default=300,
                     help="the unfused loop in configuration A (TT06Dev tolerance 0, the "
                          "factory's default deck) for this many steps")
+    ap.add_argument("--exact-fused-steps", type=int, default=1000,
+                    help="the fused FP64 engine in configuration A for this many steps")
     ap.add_argument("--mixed-steps", type=int, default=5000)
     ap.add_argument("--exact-steps", type=int, default=2000)
     ap.add_argument("--tune-steps", type=int, default=40)
@@ -404,6 +407,13 @@ def main():
         res[mode] = r
         log(f"{mode}: launch {r['launch']}, {steps} steps in {r['ms']:.1f} ms -> "
             f"{r['rate']:.3f} Gcell-steps/s ({time.perf_counter() - t:.1f} s incl. set-up)")
+    if args.exact_fused_steps > 0:
+        t = time.perf_counter()
+        r = fused_run(hip, slab, anat.ncell, "fp64_fused", args.exact_fused_steps, args.warmup,
+                      args.tune_steps, args.graph, exact=True)
+        res["fp64_fused_exact"] = r
+        log(f"fp64_fused, configuration A: launch {r['launch']}, {r['rate']:.3f} Gcell-steps/s "
+            f"({time.perf_counter() - t:.1f} s incl. set-up)")
     if args.unfused_steps > 0:
         sim = hip.Simulation(anat, W.CELLTYPES_EMI, DT, W.DIFFUSION_SCALE, stims,
                              mode="fp64_unfused")
@@ -518,6 +528,16 @@ def main():
             "note": "TT06Dev tolerance 0, mod 1 (the reaction factory's default deck): exact "
                     "functions with the device's exp/log/pow, generic gate update; the same "
                     "loop and diffusion as fp64_unfused"}
+    if "fp64_fused_exact" in res:
+        ex = res["fp64_fused_exact"]
+        out["fp64_fused_configuration_A"] = {
+            "value": ex["rate"], "unit": "Gcell-steps/s", "steps": ex["steps"],
+            "ms_per_step": ex["ms"] / ex["steps"], "launch": launch_json(ex["launch"]),
+            "note": "TT06Dev tolerance 0, mod 1 in the fused engine: bit for bit the unfused "
+                    "configuration A step"}
+        if "fp64_unfused_exact" in res:
+            out["fp64_fused_configuration_A"]["speedup_over_unfused"] = (
+                ex["rate"] / res["fp64_unfused_exact"]["rate"])
     if args.shard_steps > 0 and args.shard_parts > 1:
         sh = shards(hip, W, anat, stims, args.shard_parts, "fp64_fma", f64["launch"],
                     args.shard_steps, args.graph)

@@ -85,7 +85,8 @@ int cdh_batch_create(const cdh_batch_params *p, cdh_batch **out)
       tt06_constants(&b->K);
       b->eng = std::make_unique<Engine>(p->mode, p->dt, p->fit_l, p->fit_m, p->fit_coef,
                                         p->cell_capacity, p->box_capacity, p->slab_capacity,
-                                        p->param_capacity, p->sensors);
+                                        p->param_capacity, p->sensors, p->exact_functions,
+                                        p->jhTauSmooth);
       *out = b.release();
    });
 }
@@ -129,7 +130,9 @@ int cdh_batch_add(cdh_batch *b, const cdh_slab_desc *d, int *slab)
       CDH_REQUIRE(slab, "cdh_batch_add: null slab output");
       Engine &e = engine(b);
       std::vector<CellType> cts;
-      const SlabInput in = slab_input(d, cts);
+      SlabInput in = slab_input(d, cts);
+      // configuration A's generic gate loop has no v2 loop bounds
+      if (e.exact()) in.gate_tail_rule = 0;
       int64_t ntile = 0, nbox = 0;
       slab_extent(in, &ntile, &nbox);
       if (!e.fits(ntile, nbox, 1)) throw Error(CDH_ERR_FULL, "cdh_batch_add: arena full");

@@ -74,6 +74,7 @@ struct EngineArgs {
    // constants in device memory rather than kernel arguments: the kernel
    // would otherwise load all of them into SGPRs at entry and spill them
    const TT06Const *K;
+   const TT06ExactConst *E;     // configuration A's constants (exact engines), or null
    const double *coef;
    const ParamSet *params;
    const uint32_t *meta;        // per cell: meta_pack(pid, flags, stim mask)
@@ -234,10 +235,11 @@ __device__ __forceinline__ int engine_tile_group(const EngineArgs &a)
    return (((c << 3) + xcd) << sh) + (r - (c << sh));
 }
 
-// Buffer resource over a box so that (box index)*8 is the VGPR offset and
-// a shifted stencil offset is the SGPR offset.
+// the FP64 engine's reaction arithmetic (engine_f64.hip)
+enum EngineKind : int { kFits = 0, kFitsFma = 1, kExact = 2 };
+
 // kernels (engine_f64.hip, engine_mixed.hip); WAVES = target waves per SIMD
-template <int WAVES, bool FMA>
+template <int WAVES, int KIND>
 __global__ void k_engine_f64(EngineArgs a);
 template <int WAVES>
 __global__ void k_engine_mixed(EngineArgs a);

@@ -4,18 +4,24 @@
 // stencil and the Vm update round as the reference CPU path does, operation
 // by operation (TT06Dev_Reaction::calc, elec/TT06Dev_Reaction.cc:393-407;
 // FGRDiffusionOMP::calc, elec/FGRDiffusionOMP.cc:109-131; the integrator,
-// elec/simulationLoop.cc:221-227).  FMA = true evaluates the 33 fit
-// polynomials with fused multiply-adds (half the instructions of the
-// multiply-add pairs) and leaves everything else as it is.
+// elec/simulationLoop.cc:221-227).  KIND selects the reaction's arithmetic:
+// kFits, the 33 rational fits as the reference rounds them; kFitsFma, the
+// fit polynomials with fused multiply-adds (half the instructions of the
+// multiply-add pairs) and everything else as it is; kExact, configuration A
+// (TT06Dev tolerance 0, the factory's default deck): the exact voltage
+// functions with the device's exp, log, sqrt and pow and the generic gate
+// update (elec/TT06Func.cc:64-161), rounded as the drop-in reaction object
+// rounds them, so that the fused step equals the unfused loop bit for bit.
 #pragma clang fp contract(off)
 
 #include "engine.hpp"
+#include "tt06_exact.hpp"
 
 namespace cdh {
 
 namespace {
 
-template <bool FMA>
+template <int KIND>
 __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, int sl,
                                          const SlabView &v, int blk, int lane)
 {
@@ -60,22 +66,30 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    // always set): otherwise the compiler hoists its scalar coefficient loads
    // to the kernel entry and spills them
    double dVR = 0.0;
-   if (a.gate_mask & (1u << 12)) dVR = nongate64<FMA>(*a.K, P, a.coef, a.dt, V, st);
-   gates64<FMA>(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, st, a.gate_mask);
+   if constexpr (KIND == kExact) {
+      if (a.gate_mask & (1u << 12))
+         dVR = nongate_core64(*a.K, P, ExactVoltageFns{a.E, a.K}, a.dt, V, st);
+      if (!((meta >> 8) & kFlagGatesFixed))
+         gates_exact64(*a.E, P.s_switch, a.dt, V, st, a.gate_mask);
+   } else {
+      constexpr bool FMA = KIND == kFitsFma;
+      if (a.gate_mask & (1u << 12)) dVR = nongate64<FMA>(*a.K, P, a.coef, a.dt, V, st);
+      gates64<FMA>(a.coef, a.K->exp_coef, P.s_switch, (uint8_t)(meta >> 8), V, st, a.gate_mask);
+   }
    // Reads of the slab's descriptor after a vector store become vector loads
    // that wait for it, so the range check's limits come from the view read
    // at the start.  Since then the stores' place makes no difference: on one
    // MI355X, both instances within 0.2 % with the stores before or after the
    // range check and the stimulus (the FMA step 0.5 % faster than when the
    // limits were read here).
-   if constexpr (FMA) {
+   if constexpr (KIND == kFitsFma) {
 #pragma unroll
       for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
    }
    engine_range(a, S, v, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);
-   if constexpr (!FMA) {
+   if constexpr (KIND != kFitsFma) {
 #pragma unroll
       for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
    }
@@ -94,7 +108,7 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
 // grid covers the live slabs' tiles only: tiles of removed slabs cost nothing.
 // When the live tiles are one range (one anatomy, a batch without holes)
 // the tile is the index plus a base, with no list to read first.
-template <int WAVES, bool FMA>
+template <int WAVES, int KIND>
 __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
 {
    const int wpg = blockDim.x >> 6;
@@ -109,13 +123,15 @@ __global__ __launch_bounds__(256, WAVES) void k_engine_f64(EngineArgs a)
    if (!S.active) return;
    SlabView v;
    slab_view(S, a.parity, a.dt, v);
-   cell_f64<FMA>(a, S, sl, v, blk, lane);
+   cell_f64<KIND>(a, S, sl, v, blk, lane);
    if (blk == S.first_blk && lane == 0) slab_advance(S, a.parity, v);
 }
 
-template __global__ void k_engine_f64<2, false>(EngineArgs);
-template __global__ void k_engine_f64<3, false>(EngineArgs);
-template __global__ void k_engine_f64<2, true>(EngineArgs);
-template __global__ void k_engine_f64<3, true>(EngineArgs);
+template __global__ void k_engine_f64<2, kFits>(EngineArgs);
+template __global__ void k_engine_f64<3, kFits>(EngineArgs);
+template __global__ void k_engine_f64<2, kFitsFma>(EngineArgs);
+template __global__ void k_engine_f64<3, kFitsFma>(EngineArgs);
+template __global__ void k_engine_f64<2, kExact>(EngineArgs);
+template __global__ void k_engine_f64<3, kExact>(EngineArgs);
 
 }  // namespace cdh

@@ -84,13 +84,15 @@ __global__ __launch_bounds__(256) void k_halo(int64_t n, const uint32_t *pairs, 
 
 using KernelFn = void (*)(EngineArgs);
 
-KernelFn kernel_for(int mode, int waves)
+KernelFn kernel_for(int mode, bool exact, int waves)
 {
    if (mode == CDH_MODE_MIXED_FUSED) {
       return waves == 4 ? k_engine_mixed<4> : k_engine_mixed<3>;
    }
-   if (mode == CDH_MODE_FP64_FMA) return waves == 2 ? k_engine_f64<2, true> : k_engine_f64<3, true>;
-   return waves == 2 ? k_engine_f64<2, false> : k_engine_f64<3, false>;
+   if (exact) return waves == 2 ? k_engine_f64<2, kExact> : k_engine_f64<3, kExact>;
+   if (mode == CDH_MODE_FP64_FMA)
+      return waves == 2 ? k_engine_f64<2, kFitsFma> : k_engine_f64<3, kFitsFma>;
+   return waves == 2 ? k_engine_f64<2, kFits> : k_engine_f64<3, kFits>;
 }
 
 int min_waves(int mode) { return mode == CDH_MODE_MIXED_FUSED ? 3 : 2; }
@@ -297,14 +299,19 @@ void Engine::retire_staging()
 }
 
 Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const double *fit_coef,
-               int64_t cap_cells, int64_t cap_box, int cap_slabs, int cap_params, int sensors)
-    : mode_(mode), dt_(dt), sensors_(sensors)
+               int64_t cap_cells, int64_t cap_box, int cap_slabs, int cap_params, int sensors,
+               int exact, int jhTauSmooth)
+    : mode_(mode), dt_(dt), sensors_(sensors), exact_(exact ? 1 : 0)
 {
    CDH_REQUIRE(mode == CDH_MODE_FP64_FUSED || mode == CDH_MODE_FP64_FMA ||
                    mode == CDH_MODE_MIXED_FUSED,
                "engine: mode must be FP64 fused, FP64 FMA or mixed fused");
+   if (exact_ && mode != CDH_MODE_FP64_FUSED)
+      throw Error(kErrUnsupported,
+                  "engine: the exact functions (TT06Dev tolerance 0) run in the FP64 fused mode "
+                  "only");
    CDH_REQUIRE(dt > 0, "engine: dt must be positive");
-   CDH_REQUIRE(fit_l && fit_m && fit_coef, "engine: null fit arrays");
+   CDH_REQUIRE(exact_ || (fit_l && fit_m && fit_coef), "engine: null fit arrays");
    CDH_REQUIRE(cap_cells >= 1 && cap_cells < (int64_t(1) << 31) - 64,
                "engine: cell capacity must be 1..2^31-65");
    CDH_REQUIRE(cap_box >= 1 && cap_box < (int64_t(1) << 28), "engine: box capacity must be below 2^28");
@@ -331,7 +338,14 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    tt06_constants(&K_);
    track(dK_, 1);
    dK_.upload(&K_, 1);
-   std::vector<double> coef = scaled_fit_coefficients(fit_l, fit_m, fit_coef, dt);
+   std::vector<double> coef = exact_ ? std::vector<double>(kFitCoefTotal, 0.0)
+                                     : scaled_fit_coefficients(fit_l, fit_m, fit_coef, dt);
+   if (exact_) {
+      TT06ExactConst E;
+      tt06_exact_constants(&E, jhTauSmooth);
+      track(dE_, 1);
+      dE_.upload(&E, 1);
+   }
    track(coef_, coef.size());
    coef_.upload(coef.data(), coef.size());
    track(psets_, cap_params);
@@ -605,6 +619,7 @@ EngineArgs Engine::args(int parity) const
    a.t90 = t90_.p;
    a.counts = counts_.p;
    a.ranges = ranges_.p;
+   a.E = dE_.p;
    return a;
 }
 
@@ -613,7 +628,8 @@ void Engine::launch_step(int parity)
    const EngineArgs a = args(parity);
    if (a.ntile == 0) return;
    const dim3 grid(8 * a.groups_per_xcd), blk(cfg_.block);
-   hipLaunchKernelGGL(kernel_for(mode_, cfg_.waves), grid, blk, lds_bytes(cfg_), stream_, a);
+   hipLaunchKernelGGL(kernel_for(mode_, exact_ != 0, cfg_.waves), grid, blk, lds_bytes(cfg_),
+                      stream_, a);
    CDH_LAUNCH_CHECK();
    if (nhalo_ > 0) {
       hipLaunchKernelGGL(k_halo, dim3((unsigned)((nhalo_ + 255) / 256)), dim3(256), 0, stream_,

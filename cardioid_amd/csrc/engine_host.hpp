@@ -65,9 +65,13 @@ class Engine {
       int loop = 0;
    };
 
-   // mode: CDH_MODE_FP64_FUSED or CDH_MODE_MIXED_FUSED
+   // mode: CDH_MODE_FP64_FUSED, CDH_MODE_FP64_FMA or CDH_MODE_MIXED_FUSED.
+   // exact: configuration A (TT06Dev tolerance 0: the exact functions; mod
+   // jhTauSmooth), FP64 fused only; the fit arrays are then not used and
+   // slabs must carry no v2 loop-bound gate flags
    Engine(int mode, double dt, const int *fit_l, const int *fit_m, const double *fit_coef,
-          int64_t cap_cells, int64_t cap_box, int cap_slabs, int cap_params, int sensors);
+          int64_t cap_cells, int64_t cap_box, int cap_slabs, int cap_params, int sensors,
+          int exact = 0, int jhTauSmooth = 0);
    ~Engine();
    Engine(const Engine &) = delete;
    Engine &operator=(const Engine &) = delete;
@@ -120,6 +124,7 @@ class Engine {
    static constexpr int kSensorFields = 6;
 
    int mode() const { return mode_; }
+   bool exact() const { return exact_ != 0; }
    int64_t device_bytes() const { return bytes_; }
    int64_t cells_used() const { return tiles_.used() * kBlk; }
    int64_t box_used() const { return boxes_.used(); }
@@ -217,6 +222,8 @@ class Engine {
    DevBuf<uint8_t> apd_phase_;
    DevBuf<RangeRec> ranges_;
    DevBuf<uint32_t> counts_;      // per slab: recorded tact0, tact, trec, (unused)
+   int exact_ = 0;                 // configuration A
+   DevBuf<TT06ExactConst> dE_;     // its constants
 };
 
 // mixed-mode FP32 slots (tt06_mixed.hpp order) and the reference slot each

@@ -225,10 +225,11 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
                       p->mode == CDH_MODE_MIXED_FUSED || p->mode == CDH_MODE_FP64_FMA,
                   "cdh_sim_create: unknown mode");
       CDH_REQUIRE(p->dt > 0, "cdh_sim_create: dt must be positive");
-      if (p->exact_functions && p->mode != CDH_MODE_FP64_UNFUSED)
+      if (p->exact_functions && p->mode != CDH_MODE_FP64_UNFUSED &&
+          p->mode != CDH_MODE_FP64_FUSED)
          throw Error(kErrUnsupported,
                      "cdh_sim_create: the exact functions (TT06Dev tolerance 0) run in the "
-                     "unfused mode only");
+                     "unfused and FP64 fused modes only");
       CDH_REQUIRE(p->exact_functions || (p->fit_l && p->fit_m && p->fit_coef),
                   "cdh_sim_create: null fit arrays");
       CDH_REQUIRE(!(p->sensors & 2) || (p->sensors & 1),
@@ -265,7 +266,8 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
          in.celltypes = cts.data();
          in.scales = p->scales;
          in.threshold = p->threshold;
-         in.gate_tail_rule = p->gate_tail_rule;
+         // configuration A's generic gate loop has no v2 loop bounds
+         in.gate_tail_rule = p->exact_functions ? 0 : p->gate_tail_rule;
          in.nstim = nstim;
          in.stim = stim;
          in.check_ranges = p->check_ranges;
@@ -277,7 +279,8 @@ int cdh_sim_create(const cdh_sim_params *p, int nstim, const cdh_box_stim *stim,
          } else {
             const SlabHost h = prepare_slab(in, s->K.c9);
             s->eng = std::make_unique<Engine>(p->mode, p->dt, p->fit_l, p->fit_m, p->fit_coef,
-                                              h.S, h.nbox, 1, (int)h.psets.size(), p->sensors);
+                                              h.S, h.nbox, 1, (int)h.psets.size(), p->sensors,
+                                              p->exact_functions, p->jhTauSmooth);
             LaunchCfg c;
             c.waves = p->variant ? p->variant : 3;
             c.block = 64;

@@ -73,7 +73,8 @@ class SimParams(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("dt", C.c_double), ("fit_l", ip), ("fit_m", ip),
                 ("fit_coef", dp), ("cell_capacity", C.c_int64), ("box_capacity", C.c_int64),
-                ("slab_capacity", C.c_int), ("param_capacity", C.c_int), ("sensors", C.c_int)]
+                ("slab_capacity", C.c_int), ("param_capacity", C.c_int), ("sensors", C.c_int),
+                ("exact_functions", C.c_int), ("jhTauSmooth", C.c_int)]
 
 
 class SlabDesc(C.Structure):
@@ -504,14 +505,18 @@ class Batch:
 
     def __init__(self, dt: float, cell_capacity: int, box_capacity: int, slab_capacity: int = 256,
                  param_capacity: int = 4096, mode: str = "mixed_fused", sensors: int = 1,
-                 fits: Optional[FitSet] = None):
+                 fits: Optional[FitSet] = None, exact_functions: bool = False,
+                 jhTauSmooth: int = 0):
+        """exact_functions: TT06Dev configuration A (tolerance 0, the reaction
+        factory's default deck) in mode fp64_fused; jhTauSmooth the deck's mod."""
         self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
         fl, fm, fc = self.fits.packed()
         self._keep = [fl, fm, fc]
         p = BatchParams(mode=MODES[mode], dt=dt, fit_l=ptr(fl, ip), fit_m=ptr(fm, ip),
                         fit_coef=ptr(fc), cell_capacity=int(cell_capacity),
                         box_capacity=int(box_capacity), slab_capacity=int(slab_capacity),
-                        param_capacity=int(param_capacity), sensors=int(sensors))
+                        param_capacity=int(param_capacity), sensors=int(sensors),
+                        exact_functions=int(bool(exact_functions)), jhTauSmooth=int(jhTauSmooth))
         h = C.c_void_p()
         check(lib().cdh_batch_create(C.byref(p), C.byref(h)), "cdh_batch_create")
         self.h = h

@@ -268,7 +268,8 @@ typedef struct {
                                    the device, bit for bit the same */
    int exact_functions;         /* 1: TT06Dev configuration A (tolerance 0: the exact
                                    functions, generic gate update; fit arrays may be
-                                   NULL); CDH_MODE_FP64_UNFUSED only */
+                                   NULL); CDH_MODE_FP64_UNFUSED and CDH_MODE_FP64_FUSED
+                                   (bit for bit the same step) */
    int jhTauSmooth;             /* with exact_functions: the mod keyword (1: smoothed
                                    h and j rates) */
 } cdh_sim_params;
@@ -359,6 +360,10 @@ typedef struct {
    int slab_capacity;
    int param_capacity;          /* parameter sets; a slab takes one per (cell type, scales) */
    int sensors;                 /* bit 0: activation times; bit 1: APD90 as well */
+   int exact_functions;         /* 1: TT06Dev configuration A (tolerance 0), mode
+                                   CDH_MODE_FP64_FUSED only; fit arrays unused, the
+                                   slabs' gate_tail_rule ignored (no v2 loop bounds) */
+   int jhTauSmooth;             /* with exact_functions: the deck's mod keyword */
 } cdh_batch_params;
 
 typedef struct {

@@ -400,3 +400,36 @@ def test_cfg3_sixty_four_slabs_mixed_autotuned_graphs():
         say(f"  slab {k}: {cases[k].anatomy.nx}x{cases[k].anatomy.ny}x{cases[k].anatomy.nz}, "
             f"activated {(act >= 0).mean():.3f}, alone == in the batch: {same}")
         assert same
+
+
+def test_configuration_A_batch_alone_equals_in_batch_and_other_modes_refuse():
+    """Configuration A (the factory's default deck) in the batched engine:
+    two slabs with drug scales, each bit for bit what it gives alone after
+    400 steps; slabs described with the v2 loop-bound rule run without it,
+    as the generic gate loop has none.  The FMA and mixed engines refuse
+    the exact functions."""
+    cases = W.batched_slabs(n=2, seed=3, lo=(12, 12, 12), hi=(20, 20, 20))
+    slabs = [hip.Slab(c.anatomy, ["endoCellML"], W.DIFFUSION_SCALE, c.stimuli, scales=c.scales)
+             for c in cases]
+    ext = [s.extent() for s in slabs]
+    cells, boxes = sum(e[0] for e in ext), sum(e[1] for e in ext)
+
+    def run(which):
+        b = hip.Batch(DT, cells, boxes, slab_capacity=2, mode="fp64_fused", exact_functions=True,
+                      jhTauSmooth=1)
+        ids = [b.add(slabs[k]) for k in which]
+        b.run(400)
+        out = {k: (b.vm(i), b.state(i), b.activation(i)[1]) for k, i in zip(which, ids)}
+        b.close()
+        return out
+
+    both = run([0, 1])
+    for k in (0, 1):
+        alone = run([k])[k]
+        say(f"  configuration A slab {k}: {slabs[k].ncell} cells, activated "
+            f"{np.mean(alone[2] >= 0):.3f}")
+        for x, y in zip(alone, both[k]):
+            assert np.array_equal(x, y)
+    for mode in ("fp64_fma", "mixed_fused"):
+        with pytest.raises(hip.CardioidHipError, match="exact"):
+            hip.Batch(DT, cells, boxes, slab_capacity=2, mode=mode, exact_functions=True)

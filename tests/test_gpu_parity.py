@@ -315,9 +315,10 @@ def test_cfg0_full_size_matches_oracle():
 def test_cfg0_configuration_A_matches_oracle(mod, size):
     """cfg0 with the factory's default reaction deck (tolerance 0: exact
     functions, generic gate update), in the reference's loop over the drop-in
-    objects: 64^3 endocardial brick for mod 0 (the default), a 24^3 brick for
-    mod 1; 1000 steps.  Vm within 1e-9 relative and activation times
-    identical to the oracle's configuration A."""
+    objects and in the fused engine: 64^3 endocardial brick for mod 0 (the
+    default), a 24^3 brick for mod 1; 1000 steps.  Vm within 1e-9 relative
+    and activation times identical to the oracle's configuration A, and the
+    fused step bit for bit the unfused one."""
     import oracle_lib as OL
     say(f"cfg0 configuration A, brick {size}^3, mod={mod}, 1000 steps")
     a = W.brick(size, size, size)
@@ -327,21 +328,28 @@ def test_cfg0_configuration_A_matches_oracle(mod, size):
     for k in range(4):
         ref.run(250)
         say(f"  oracle {250 * (k + 1)} steps")
-    sim = hip.Simulation(a, ["endoCellML"], DT, W.DIFFUSION_SCALE, stims, mode="fp64_unfused",
-                         exact_functions=True, jhTauSmooth=mod)
-    sim.run(1000)
-    vm, act = sim.vm(), sim.activation()
     want_vm, want_act = ref.vm(), ref.activation()
-    assert abs(sim.time - ref.time) == 0.0
-    e = relerr(vm, want_vm)
-    say(f"  Vm max rel {e.max():.3e} bitwise {np.mean(vm == want_vm):.4f}")
-    assert e.max() < 1e-9
-    for k, nm in enumerate(("tact0", "tact", "trec")):
-        same = act[k] == want_act[k]
-        say(f"  {nm} identical {same.mean():.4f}, activated {np.mean(want_act[k] >= 0):.3f}")
-        assert same.all(), nm
+    out = {}
+    # the reference's loop over the drop-in objects, and the fused engine
+    for mode in ("fp64_unfused", "fp64_fused"):
+        sim = hip.Simulation(a, ["endoCellML"], DT, W.DIFFUSION_SCALE, stims, mode=mode,
+                             exact_functions=True, jhTauSmooth=mod)
+        sim.run(1000)
+        vm, act = sim.vm(), sim.activation()
+        assert abs(sim.time - ref.time) == 0.0
+        e = relerr(vm, want_vm)
+        say(f"  {mode}: Vm max rel {e.max():.3e} bitwise {np.mean(vm == want_vm):.4f}")
+        assert e.max() < 1e-9
+        for k, nm in enumerate(("tact0", "tact", "trec")):
+            same = act[k] == want_act[k]
+            say(f"  {nm} identical {same.mean():.4f}, activated {np.mean(want_act[k] >= 0):.3f}")
+            assert same.all(), nm
+        out[mode] = (vm, sim.state())
+        sim.close()
+    # the fused step rounds as the drop-in objects do: bit for bit the same
+    assert np.array_equal(out["fp64_unfused"][0], out["fp64_fused"][0])
+    assert np.array_equal(out["fp64_unfused"][1], out["fp64_fused"][1])
     assert (want_act[1] >= 0).mean() > 0.5
-    sim.close()
     ref.close()
 
 
