@@ -137,6 +137,16 @@ std::vector<T> block(const std::vector<T> &soa, int nq, int64_t S, int64_t strid
    return b;
 }
 
+// the mixed engine's 9 weight words of S cells, 9 plain rows of S, into
+// their load groups (grp_index9)
+std::vector<uint32_t> group9(const std::vector<uint32_t> &soa, int64_t S)
+{
+   std::vector<uint32_t> b((size_t)9 * S);
+   for (int q = 0; q < 9; q++)
+      for (int64_t k = 0; k < S; k++) b[grp_index9(q, k)] = soa[q * S + k];
+   return b;
+}
+
 template <class T>
 void download(const T *dev, T *host, size_t count)
 {
@@ -500,7 +510,7 @@ int Engine::add(const SlabHost &h)
          put(Aw_.p + c0 * 18, block(h.Aw, 18, S, S));
       } else {
          put(Aref_.p + t0 * 18, h.Aref);
-         put(Aw16_.p + c0 * 9, block(h.Aw16, 9, S, S));
+         put(Aw16_.p + c0 * 9, group9(h.Aw16, S));
       }
       put(box_[0].p + b0, h.box0);
       put(box_[1].p + b0, h.box0);

@@ -41,7 +41,8 @@ struct StencilArgs {
    const uint32_t *bi;          // box index per cell
    const double *box;           // Vm in the padded box
    const float *Aw;             // off-centre weights, NodeLocation order 1..18, in 64-cell
-                                // tiles (blk_index); the centre weight is minus their sum
+                                // tiles in load groups (grp_index18); the centre weight is
+                                // minus their sum
    int off[19];                 // NodeLocation offsets in the box
    double scale;                // diffusionScale
    double *dVm;

@@ -217,9 +217,27 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
    // on the weights for the centre-weight sum, which costs a third round
    // trip to memory per wave (4 % of the kernel's time).
    const uint32_t b = a.bi[i];
+   // the 18 weights in five loads (grp_index18): four of 16 bytes and one
+   // of 8 bytes per lane.  The kernel's address unit is busy for 95 % of its
+   // time (rocprof TA_TA_BUSY), so fewer load instructions help: 365 against
+   // 373 us per call on the 20.1 M-cell shell with one load per weight, bit
+   // for bit the same
+   const float *tw = a.Aw + (i >> 6) * (18 * kBlk);
+   const int lane = (int)(i & (kBlk - 1));
    float w[18];
 #pragma unroll
-   for (int k = 0; k < 18; k++) w[k] = __builtin_nontemporal_load(a.Aw + blk_index(k, i, 18));
+   for (int g = 0; g < 4; g++) {
+      const f32x4 q = __builtin_nontemporal_load((const f32x4 *)(tw + g * 4 * kBlk) + lane);
+      w[4 * g] = q.x;
+      w[4 * g + 1] = q.y;
+      w[4 * g + 2] = q.z;
+      w[4 * g + 3] = q.w;
+   }
+   {
+      const f32x2 q = __builtin_nontemporal_load((const f32x2 *)(tw + 16 * kBlk) + lane);
+      w[16] = q.x;
+      w[17] = q.y;
+   }
    __builtin_amdgcn_sched_barrier(0);
    const double *phi = a.box + b;
    double v[19];

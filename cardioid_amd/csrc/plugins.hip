@@ -404,10 +404,11 @@ int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
          dT.upload(tissue.data(), nbox);
          dS.upload(sig.data(), 6 * nbox);
          dB.upload(bidx.data(), p->nLocal);
-         // weights in 64-cell tiles (field k of cell c at blk_index(k, c, 18)):
-         // a wavefront's 18 weight rows are one contiguous 4.5 KB run
+         // weights in 64-cell tiles, in the stencil's load groups
+         // (grp_index18): a wavefront's 18 weights are one contiguous 4.5 KB
+         // run that five load instructions bring in
          device_fgr_weights(nullptr, dT.p, dS.p, p->nLocal, dB.p, fgr::weight_geom(g, off),
-                            d->A0.p, d->Aw.p, d->stride, true);
+                            d->A0.p, d->Aw.p, d->stride, kWeightsGrouped);
          CDH_HIP(hipDeviceSynchronize());
       }
       StencilArgs &a = d->args;

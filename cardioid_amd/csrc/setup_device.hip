@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_fibre_sigma_int(int64_t n, const int
 __global__ __launch_bounds__(kBlock) void k_fgr_weights(const uint8_t *tissue, const double *sigma,
                                                         int64_t n, const int64_t *bidx,
                                                         fgr::WeightGeom G, double *A0, float *Aw,
-                                                        int64_t stride, int blocked)
+                                                        int64_t stride, int layout)
 {
    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
    if (c >= n) return;
@@ -84,7 +84,10 @@ __global__ __launch_bounds__(kBlock) void k_fgr_weights(const uint8_t *tissue, c
    fgr::cell_weights(tissue, sigma, bidx[c], G, w);
    A0[c] = fgr::centre_weight(w);
 #pragma unroll
-   for (int k = 1; k < 19; k++) Aw[blocked ? blk_index(k - 1, c, 18) : (k - 1) * stride + c] = w[k];
+   for (int k = 1; k < 19; k++)
+      Aw[layout == kWeightsGrouped ? grp_index18(k - 1, c)
+         : layout == kWeightsTiled ? blk_index(k - 1, c, 18)
+                                   : (k - 1) * stride + c] = w[k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter_box(int64_t n, const uint32_t *bi, uint32_t b0,
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const float *W, int64_t S, i
       const double d = (double)w * up - (double)ref;
       word |= (uint32_t)fgr::half_bits(fgr::half_rne(d)) << (16 * h);
    }
-   Aw16[blk_index(p, c, 9)] = word;
+   Aw16[grp_index9(p, c)] = word;
 }
 
 }  // namespace
@@ -277,11 +280,11 @@ double *noalloc_scratch(hipStream_t s)
 
 void device_fgr_weights(hipStream_t stream, const uint8_t *tissue, const double *sigma,
                         int64_t ncell, const int64_t *bidx, const fgr::WeightGeom &G, double *A0,
-                        float *Aw, int64_t stride, bool blocked)
+                        float *Aw, int64_t stride, int layout)
 {
    if (ncell == 0) return;
    hipLaunchKernelGGL(k_fgr_weights, dim3(grid_for(ncell)), dim3(kBlock), 0, stream, tissue,
-                      sigma, ncell, bidx, G, A0, Aw, stride, (int)blocked);
+                      sigma, ncell, bidx, G, A0, Aw, stride, layout);
    CDH_LAUNCH_CHECK();
 }
 

@@ -54,10 +54,12 @@ __host__ __device__ inline int compact_weight_exponent(float m)
 
 // precomputeCoefficients on the device for ncell cells at box points
 // bidx[] of a box laid out as G.off describes (device arrays; asynchronous
-// on stream): A0[c] and Aw[(k-1)*stride + c], or with `blocked` the 64-cell
-// tile layout Aw[blk_index(k-1, c, 18)]
+// on stream): A0[c] and, by layout, Aw[(k-1)*stride + c] (kWeightsPlain),
+// Aw[blk_index(k-1, c, 18)] (kWeightsTiled) or Aw[grp_index18(k-1, c)]
+// (kWeightsGrouped)
+enum WeightLayout : int { kWeightsPlain = 0, kWeightsTiled = 1, kWeightsGrouped = 2 };
 void device_fgr_weights(hipStream_t stream, const uint8_t *tissue, const double *sigma,
                         int64_t ncell, const int64_t *bidx, const fgr::WeightGeom &G, double *A0,
-                        float *Aw, int64_t stride, bool blocked = false);
+                        float *Aw, int64_t stride, int layout = kWeightsPlain);
 
 }  // namespace cdh

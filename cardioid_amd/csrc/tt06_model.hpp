@@ -92,6 +92,32 @@ __host__ __device__ inline int64_t blk_index(int64_t q, int64_t i, int nq)
    return blk_base(i, nq) + (int64_t)kBlk * q;
 }
 
+// vector types for loads of several fields per lane
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// The drop-in stencil's 18 weights per cell, in 64-cell tiles with the
+// fields in groups that one load instruction each brings in: fields 0-3,
+// 4-7, 8-11 and 12-15 as 16 bytes per lane, fields 16-17 as 8 bytes per
+// lane.  A tile's groups are one contiguous 4.5 KB run, as in blk_index.
+__host__ __device__ inline int64_t grp_index18(int k, int64_t i)
+{
+   const int64_t base = (i >> 6) * (18 * kBlk);
+   const int64_t lane = i & (kBlk - 1);
+   return k < 16 ? base + (k >> 2) * (4 * kBlk) + 4 * lane + (k & 3)
+                 : base + 16 * kBlk + 2 * lane + (k - 16);
+}
+
+// The mixed engine's 9 weight words per cell the same way: words 0-3 and
+// 4-7 as 16 bytes per lane, word 8 as 4 bytes per lane.
+__host__ __device__ inline int64_t grp_index9(int k, int64_t i)
+{
+   const int64_t base = (i >> 6) * (9 * kBlk);
+   const int64_t lane = i & (kBlk - 1);
+   return k < 8 ? base + (k >> 2) * (4 * kBlk) + 4 * lane + (k & 3) : base + 8 * kBlk + lane;
+}
+
 // The mixed engine's state arrays: per slab, runs of kRun cells, field-major
 // inside a run (field q of the run's cells one contiguous stretch), the last
 // run as wide as the cells left (a multiple of 64, as slabs are).  Measured

@@ -10,7 +10,8 @@ the launch configuration: register target, workgroup size, occupancy cap), or
 fp64_unfused (the reference loop over the drop-in objects: reaction, scatter,
 FGR stencil, integrator and sensor kernels; WAVES/BLOCK/OCC ignored), or
 fp64_unfused_exact (the same loop with the reaction factory's default deck,
-TT06Dev tolerance 0: the exact functions).  Meant
+TT06Dev tolerance 0: the exact functions), or fp64_fused_exact (that deck in
+the fused FP64 engine).  Meant
 to run under rocprofv3 (kernel trace, or one --pmc counter per run).
 """
 from __future__ import annotations
@@ -53,8 +54,10 @@ def main(argv):
         slabs = [hip.Slab(c.anatomy, ["endoCellML"], W.DIFFUSION_SCALE, c.stimuli, scales=c.scales)
                  for c in W.batched_slabs()]
     ext = [s.extent() for s in slabs]
+    exact = mode == "fp64_fused_exact"
     b = hip.Batch(0.02, sum(e[0] for e in ext), sum(e[1] for e in ext), slab_capacity=len(slabs),
-                  mode=mode)
+                  mode="fp64_fused" if exact else mode, exact_functions=exact,
+                  jhTauSmooth=1 if exact else 0)
     for s in slabs:
         b.add(s)
     b.set_launch(waves, block, occ)

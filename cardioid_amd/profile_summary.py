@@ -53,6 +53,53 @@ def _pick(d, kernel):
     return keys[0]
 
 
+# counters kept per CU (one instance each), whose _sum over the CUs divided
+# by the CU count and the kernel's cycles is a busy or stall fraction
+_UNIT_FRACTIONS = ("TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "TD_TC_STALL_sum",
+                   "TCP_PENDING_STALL_CYCLES_sum", "TA_DATA_STALLED_BY_TC_CYCLES_sum",
+                   "TA_ADDR_STALLED_BY_TC_CYCLES_sum")
+# counters reported per 64-cell tile
+_UNIT_PER_TILE = ("TCP_TOTAL_CACHE_ACCESSES_sum", "TD_LOAD_WAVEFRONT_sum", "TCP_TCC_READ_REQ_sum",
+                  "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR")
+
+
+def units_section(a, per):
+    """Texture address (TA), data (TD) and L1 (TCP) counters of the kernel
+    and of the calibration copy: busy and stall fractions of the kernel's
+    cycles (GRBM_GUI_ACTIVE counts every XCD, so one XCD's cycles are it
+    divided by the XCD count), and accesses per 64-cell tile."""
+    cols = (a.kernel, COPY_KERNEL)
+    vals = {}
+    for d in a.units:
+        c = _db(d)
+        for (name,) in c.execute("select distinct counter_name from counters_collection"):
+            m = counter_means(d, name)
+            vals[name] = {k: m[_pick(m, k)][0] for k in cols if any(k in x for x in m)}
+    lines = ["## Memory units (TA, TD, TCP)", "", "Commands:", ""]
+    lines += [f"    {c}" for c in a.cmd_units]
+    lines += ["", f"Per dispatch. Fractions: counter / ({a.cus} CUs x GRBM_GUI_ACTIVE / {a.xcds} "
+              "XCDs); per tile: counter / (tissue cells / 64) per launch.", "",
+              f"| counter | `{a.kernel}` | `{COPY_KERNEL}` |", "|---|---|---|"]
+    grbm = vals.get("GRBM_GUI_ACTIVE", {})
+    tiles = a.cells / 64.0 / per
+    for name, v in vals.items():
+        row = []
+        for k in cols:
+            x = v.get(k)
+            if x is None:
+                row.append("-")
+            elif name in _UNIT_FRACTIONS and grbm.get(k):
+                row.append(f"{x:.4g} (**{x / (a.cus * grbm[k] / a.xcds):.2f}** busy)"
+                           if "BUSY" in name else
+                           f"{x:.4g} ({x / (a.cus * grbm[k] / a.xcds):.2f} of the cycles)")
+            elif name in _UNIT_PER_TILE and k == a.kernel:
+                row.append(f"{x:.4g} ({x / tiles:.1f} per tile)")
+            else:
+                row.append(f"{x:.4g}")
+        lines.append(f"| {name} | {row[0]} | {row[1]} |")
+    return lines + [""]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--kernel", required=True)
@@ -78,6 +125,12 @@ def main(argv=None):
     ap.add_argument("--flops", type=float, default=1060.0,
                     help="algorithmic floating-point operations per cell-step (FMA = 2); "
                          "0 leaves the flop fraction out")
+    ap.add_argument("--units", action="append", default=[],
+                    help="a rocprofv3 --pmc run of texture-unit / L1 counters (repeatable); "
+                         "busy fractions need GRBM_GUI_ACTIVE in the same run")
+    ap.add_argument("--cmd-units", action="append", default=[])
+    ap.add_argument("--cus", type=int, default=256)
+    ap.add_argument("--xcds", type=int, default=8)
     ap.add_argument("--peak-tflops", type=float, default=78.6)
     ap.add_argument("--simds", type=int, default=1024)
     ap.add_argument("--clock-ghz", type=float, default=2.4)
@@ -159,6 +212,8 @@ def main(argv=None):
                    valu_instructions_per_tile=insts / tiles, valu_issue_frac=issue,
                    flop_per_cell_step=a.flops, flop_frac=flop_frac,
                    peak_tflops=a.peak_tflops, clock_ghz=a.clock_ghz)
+    if a.units:
+        lines += units_section(a, per)
     with open(a.out + ".md", "w") as fh:
         fh.write("\n".join(lines))
     with open(a.out + "_traffic.json", "w") as fh:
