@@ -323,8 +323,9 @@ class Simulation:
                  setup_on_host: bool = False, exact_functions: bool = False,
                  jhTauSmooth: int = 0):
         """exact_functions: TT06Dev configuration A (tolerance 0: the exact
-        TT06 functions, generic gate update; unfused mode only), with
-        jhTauSmooth the deck's mod keyword."""
+        TT06 functions, generic gate update; modes fp64_unfused and
+        fp64_fused, bit for bit alike), with jhTauSmooth the deck's mod
+        keyword."""
         L = lib()
         self.fits = fits or read_fit_file(DEFAULT_FIT_FILE)
         fl, fm, fc = self.fits.packed()
