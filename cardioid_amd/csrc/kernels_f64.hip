@@ -15,6 +15,7 @@
 #pragma clang fp contract(off)
 
 #include "kernels.hpp"
+#include "row_shift.hpp"
 #include "tt06_exact.hpp"
 #include "tt06_f64.hpp"
 
@@ -203,6 +204,18 @@ __global__ __launch_bounds__(256) void k_scatter4_f64(int n, const uint32_t *__r
    }
 }
 
+// the drop-in box's neighbour loads, for gather_rows
+struct DropInBox {
+   const double *box;
+   const int *off;
+   __device__ __forceinline__ double at(uint32_t b, int k) const { return box[(int64_t)b + off[k]]; }
+   __device__ __forceinline__ uint32_t rel(int k) const { return (uint32_t)off[k]; }
+   __device__ __forceinline__ double at_rel(uint32_t b, uint32_t r) const
+   {
+      return box[(int64_t)b + (int32_t)r];
+   }
+};
+
 // Blocks are dealt round-robin to the 8 XCDs; each XCD takes a contiguous
 // range of cells, so that its L2 holds the box rows and planes they read.
 __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
@@ -210,19 +223,24 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
    const int xcd = blockIdx.x & 7;
    const int64_t i =
        ((int64_t)xcd * a.groups_per_xcd + (blockIdx.x >> 3)) * blockDim.x + threadIdx.x;
-   if (i >= a.n) return;
+   // a wavefront past the last cell has nothing to do; in the last one,
+   // lanes past it work on the last cell and store nothing, so that every
+   // lane takes part in the lane exchanges of the neighbour loads
+   if ((i & ~(int64_t)(kBlk - 1)) >= a.n) return;
+   const bool act = i < a.n;
+   const int64_t ic = act ? i : a.n - 1;
    // Every load is issued before any arithmetic: the box index, then the
-   // weights (they stream while the index arrives), then the 19 neighbours.
+   // weights (they stream while the index arrives), then the neighbours.
    // Left to itself the compiler interleaves the neighbour loads with waits
    // on the weights for the centre-weight sum, which costs a third round
    // trip to memory per wave (4 % of the kernel's time).
-   const uint32_t b = a.bi[i];
+   const uint32_t b = a.bi[ic];
    // the 18 weights in five loads (grp_index18): four of 16 bytes and one
-   // of 8 bytes per lane.  The kernel's address unit is busy for 95 % of its
-   // time (rocprof TA_TA_BUSY), so fewer load instructions help: 365 against
-   // 373 us per call on the 20.1 M-cell shell with one load per weight, bit
-   // for bit the same
-   const float *tw = a.Aw + (i >> 6) * (18 * kBlk);
+   // of 8 bytes per lane.  The kernel's texture units are busy for 95 % of
+   // its time (rocprof TA_TA_BUSY, TD_TD_BUSY), so fewer load instructions
+   // help: 365 against 373 us per call on the 20.1 M-cell shell with one
+   // load per weight, bit for bit the same
+   const float *tw = a.Aw + (ic >> 6) * (18 * kBlk);
    const int lane = (int)(i & (kBlk - 1));
    float w[18];
 #pragma unroll
@@ -239,11 +257,10 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
       w[17] = q.y;
    }
    __builtin_amdgcn_sched_barrier(0);
-   const double *phi = a.box + b;
+   // the 19 neighbours: nine loads, the ten x-1 / x+1 values from the
+   // neighbouring lanes where they hold them (row_shift.hpp)
    double v[19];
-   v[0] = phi[0];
-#pragma unroll
-   for (int k = 1; k < 19; k++) v[k] = phi[a.off[k]];
+   gather_rows<kRowsX>(DropInBox{a.box, a.off}, b, act, lane, v);
    __builtin_amdgcn_sched_barrier(0);
    // the weights stream once per step; the centre weight is minus their sum
    // in order, in double (FGRDiffusionOMP.cc:214-219), recomputed bit for bit
@@ -253,7 +270,7 @@ __global__ __launch_bounds__(256) void k_fgr_calc_f64(StencilArgs a)
    double tmp = A0 * v[0];
 #pragma unroll
    for (int k = 1; k < 19; k++) tmp += (double)w[k - 1] * v[k];
-   __builtin_nontemporal_store(tmp * a.scale, a.dVm + i);
+   if (act) __builtin_nontemporal_store(tmp * a.scale, a.dVm + i);
 }
 
 __global__ __launch_bounds__(256) void k_integrate_f64(int n, double dt, double *__restrict__ Vm,

@@ -12,6 +12,7 @@
 #include "hip_util.hpp"
 #include "setup_device.hpp"
 #include "kernels.hpp"
+#include "row_shift.hpp"
 #include "setup.hpp"
 #include "tt06_f64.hpp"
 
@@ -376,6 +377,13 @@ int cdh_fgr_create(const cdh_fgr_params *p, cdh_fgr **out)
       };
       int64_t off[19];
       fgr_offsets_strided(1, g.bx, (int64_t)g.bx * g.by, off);
+      // the stencil takes each row's x-1 and x+1 values from the
+      // neighbouring lanes (row_shift.hpp): they sit one box index either
+      // side of the row centre's
+      for (int r = 0; r < 5; r++)
+         CDH_REQUIRE(off[kRowsX.back[r]] == off[kRowsX.centre[r]] - 1 &&
+                         off[kRowsX.on[r]] == off[kRowsX.centre[r]] + 1,
+                     "cdh_fgr_create: stencil rows are not along x");
       std::vector<int64_t> bidx(ntot);
       std::vector<uint8_t> tissue(nbox, 0);
       std::vector<double> sig(6 * nbox, 0.0);
