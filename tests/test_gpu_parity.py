@@ -216,21 +216,34 @@ def test_reaction_configuration_A_matches_oracle(name, mod):
     assert (got_st.reshape(19, n)[7] != init.reshape(19, n)[7]).all()
 
 
-def test_fgr_plugin_matches_oracle_with_remote_cells():
+@pytest.mark.parametrize("cell_order", ["z_fastest", "gid"])
+def test_fgr_plugin_matches_oracle_with_remote_cells(cell_order):
+    """The drop-in stencil on a ragged anatomy with remote cells, bitwise
+    against the oracle.  In z-fastest cell order nearly every wavefront
+    leaves its x rows at every lane and loads its x-1 / x+1 values
+    directly; in gid order (x fastest) the rows run along the lanes and end
+    at remote or missing points, whose values the wavefront's shared end
+    load brings in."""
     import oracle_lib as OL
     from test_oracle_fgr import apply, build, spd_tensor
-    say("fgr plugin")
+    say(f"fgr plugin, {cell_order} order")
     L = hip.lib()
     rng = np.random.default_rng(3)
-    X, Y, Z = 14, 12, 10
-    t = (rng.random((X, Y, Z)) < 0.75).astype(np.uint8)
+    # gid order: rows of about 30 local cells with few holes, so that most
+    # wavefronts take the shared end load, which then also brings in the
+    # remote values at the rows' ends
+    X, Y, Z, dens, xl = (14, 12, 10, 0.75, 6) if cell_order == "z_fastest" else (40, 9, 8, 0.97, 30)
+    t = (rng.random((X, Y, Z)) < dens).astype(np.uint8)
     s6 = spd_tensor(rng, X * Y * Z).reshape(X, Y, Z, 6) * 1e-3
-    # anatomy cells: local = tissue with x <= 6, remote = tissue at x == 7
+    # anatomy cells: local = tissue with x <= xl, remote = tissue at x == xl + 1
     cells = np.argwhere(t.astype(bool))
     gid = cells[:, 0] + X * (cells[:, 1] + Y * cells[:, 2])
-    local = cells[:, 0] <= 6
-    remote = cells[:, 0] == 7
-    order = np.concatenate([np.nonzero(local)[0], np.nonzero(remote)[0]])
+    local = cells[:, 0] <= xl
+    remote = cells[:, 0] == xl + 1
+    li, ri = np.nonzero(local)[0], np.nonzero(remote)[0]
+    if cell_order == "gid":
+        li, ri = li[np.argsort(gid[li])], ri[np.argsort(gid[ri])]
+    order = np.concatenate([li, ri])
     g = gid[order].astype(np.int64)
     sig = s6[cells[order, 0], cells[order, 1], cells[order, 2]].reshape(-1).copy()
     ctype = np.ones(order.size, np.int32)
@@ -250,7 +263,7 @@ def test_fgr_plugin_matches_oracle_with_remote_cells():
         got = d_out.cpu().numpy()
     finally:
         L.cdh_fgr_destroy(h)
-    # oracle on the whole anatomy; local cells' stencils only reach x <= 7
+    # oracle on the whole anatomy; local cells' stencils only reach x <= xl + 1
     B = build(t, s6)
     bx, by, bz = B["shape"]
     phi = np.zeros((bx, by, bz))
