@@ -327,8 +327,14 @@ Engine::Engine(int mode, double dt, const int *fit_l, const int *fit_m, const do
    CDH_REQUIRE(cap_box >= 1 && cap_box < (int64_t(1) << 28), "engine: box capacity must be below 2^28");
    CDH_REQUIRE(cap_slabs >= 1 && cap_slabs < kNoSlab, "engine: slab capacity must be 1..65534");
    CDH_REQUIRE(cap_params >= 1, "engine: parameter-set capacity must be positive");
-   cfg_.waves = mode == CDH_MODE_MIXED_FUSED ? 3 : 3;
-   cfg_.block = 256;
+   // until a set_launch or an autotune: what the autotuner picks on the
+   // 20.1 M-cell shell (one box's table: FP64 FMA 1.477 ms per step with
+   // 64-thread workgroups against 1.556 with 256; mixed 0.935 with 128 and
+   // runs of 16 tile groups per XCD against 0.950 with 256, contiguous)
+   const bool mixed = mode == CDH_MODE_MIXED_FUSED;
+   cfg_.waves = mixed ? 3 : 2;
+   cfg_.block = mixed ? 128 : 64;
+   cfg_.xcd = mixed ? 4 : -1;
    const int64_t ntile = (cap_cells + kBlk - 1) / kBlk;
    const int64_t S = ntile * kBlk;
    tiles_ = RangeAlloc(ntile);
