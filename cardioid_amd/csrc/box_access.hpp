@@ -49,6 +49,28 @@ __device__ __forceinline__ void stream_store_f64(double v, __amdgpu_buffer_rsrc_
    typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, kNT);
 }
+// two adjacent FP64 fields of a cell (a pair of run_pair_index), 16 bytes
+__device__ __forceinline__ void stream_load_f64x2(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                  uint32_t soff, double &a, double &b)
+{
+   typedef unsigned int u32x4 __attribute__((__vector_size__(4 * sizeof(unsigned int))));
+   const u32x4 v =
+       __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kNT));
+   a = __builtin_bit_cast(double, ((uint64_t)v[1] << 32) | v[0]);
+   b = __builtin_bit_cast(double, ((uint64_t)v[3] << 32) | v[2]);
+}
+__device__ __forceinline__ void stream_store_f64x2(double a, double b, __amdgpu_buffer_rsrc_t r,
+                                                   uint32_t voff, uint32_t soff)
+{
+   typedef unsigned int u32x4 __attribute__((__vector_size__(4 * sizeof(unsigned int))));
+   const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+   u32x4 v;
+   v[0] = (uint32_t)ua;
+   v[1] = (uint32_t)(ua >> 32);
+   v[2] = (uint32_t)ub;
+   v[3] = (uint32_t)(ub >> 32);
+   __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, kNT);
+}
 __device__ __forceinline__ float stream_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                                  uint32_t soff)
 {

@@ -57,12 +57,20 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    for (int k = 1; k < 19; k++) tmp += (double)w[k - 1] * box_load(R, b8, S.soff[k]);
    const double dVD = tmp * v.scale;
 
-   // states stream once per step (non-temporal); the tile's base address is
-   // wave-uniform
-   double *sp = a.state + (int64_t)blk * (N_STATE * kBlk) + lane;
+   // states stream once per step (non-temporal), in the slab's runs of kRun
+   // cells with the fields in pairs (run_pair_index): a buffer over the
+   // run's base, the pair's offset in an SGPR, the lane's cell in a VGPR
+   const int first = __builtin_amdgcn_readfirstlane(S.first_blk);
+   const int64_t k0 = (int64_t)(blk - first) * kBlk;
+   const int64_t r0 = k0 & ~(kRun - 1);
+   const uint32_t rw =
+       (uint32_t)run_width(k0, (int64_t)__builtin_amdgcn_readfirstlane(S.ntile) * kBlk);
+   const __amdgpu_buffer_rsrc_t RS = stream_rsrc(a.state + ((int64_t)first * kBlk + r0) * N_STATE);
+   const uint32_t lc = (uint32_t)(k0 - r0 + lane);
    double st[N_STATE];
 #pragma unroll
-   for (int s = 0; s < N_STATE; s++) st[s] = __builtin_nontemporal_load(sp + kBlk * s);
+   for (int s = 0; s + 1 < N_STATE; s += 2) stream_load_f64x2(RS, 16u * lc, 8u * rw * s, st[s], st[s + 1]);
+   st[N_STATE - 1] = stream_load_f64(RS, 8u * lc, 8u * rw * (N_STATE - 1));
    const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
    // the non-gate update in a basic block of its own (gate_mask bit 12,
    // always set): otherwise the compiler hoists its scalar coefficient loads
@@ -86,14 +94,18 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
    // limits were read here).
    if constexpr (KIND == kFitsFma) {
 #pragma unroll
-      for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
+      for (int s = 0; s + 1 < N_STATE; s += 2)
+         stream_store_f64x2(st[s], st[s + 1], RS, 16u * lc, 8u * rw * s);
+      stream_store_f64(st[N_STATE - 1], RS, 8u * lc, 8u * rw * (N_STATE - 1));
    }
    engine_range(a, S, v, sl, a.parity, live, lane, i, V, dVR, dVD);
    const double iStim = slab_stim(S, v, a.parity, meta);
    const double Vn = V + a.dt * (dVR + dVD + iStim);
    if constexpr (KIND != kFitsFma) {
 #pragma unroll
-      for (int s = 0; s < N_STATE; s++) __builtin_nontemporal_store(st[s], sp + kBlk * s);
+      for (int s = 0; s + 1 < N_STATE; s += 2)
+         stream_store_f64x2(st[s], st[s + 1], RS, 16u * lc, 8u * rw * s);
+      stream_store_f64(st[N_STATE - 1], RS, 8u * lc, 8u * rw * (N_STATE - 1));
    }
    uint32_t ev = 0;
    if (live) {

@@ -98,16 +98,6 @@ KernelFn kernel_for(int mode, bool exact, int waves)
 int min_waves(int mode) { return mode == CDH_MODE_MIXED_FUSED ? 3 : 2; }
 int max_waves(int mode) { return mode == CDH_MODE_MIXED_FUSED ? 4 : 3; }
 
-// a slab's [nq][n] array (slab order) from its blocked device range
-template <class T>
-std::vector<T> unblock(const std::vector<T> &b, int nq, int64_t n)
-{
-   std::vector<T> out((size_t)nq * n);
-   for (int q = 0; q < nq; q++)
-      for (int64_t k = 0; k < n; k++) out[q * n + k] = b[blk_index(q, k, nq)];
-   return out;
-}
-
 // the same for the mixed engine's states: a device range of S cells in runs
 // (run_index)
 template <class T>
@@ -494,7 +484,10 @@ int Engine::add(const SlabHost &h)
       for (int64_t k = 0; k < S; k++) bi[k] = (uint32_t)(b0 + h.bi[k]);
       put(bi_.p + c0, bi);
       if (mode_ != CDH_MODE_MIXED_FUSED) {
-         put(state_.p + c0 * N_STATE, block(h.state, N_STATE, S, S));
+         std::vector<double> b((size_t)N_STATE * S);
+         for (int q = 0; q < N_STATE; q++)
+            for (int64_t k = 0; k < S; k++) b[run_pair_index(q, k, N_STATE, S)] = h.state[q * S + k];
+         put(state_.p + c0 * N_STATE, b);
       } else {
          std::vector<float> s32((size_t)kMixed32 * S);
          std::vector<double> s64((size_t)2 * S);
@@ -902,7 +895,10 @@ std::vector<double> Engine::state19(int id)
    if (mode_ != CDH_MODE_MIXED_FUSED) {
       std::vector<double> b((size_t)N_STATE * S);
       download(state_.p + c0 * N_STATE, b.data(), b.size());
-      return unblock(b, N_STATE, n);
+      std::vector<double> out((size_t)N_STATE * n);
+      for (int q = 0; q < N_STATE; q++)
+         for (int64_t k = 0; k < n; k++) out[q * n + k] = b[run_pair_index(q, k, N_STATE, S)];
+      return out;
    }
    std::vector<float> b32((size_t)kMixed32 * S);
    std::vector<double> b64((size_t)2 * S);
@@ -936,7 +932,7 @@ void Engine::set_state19(int id, const std::vector<double> &st, const std::vecto
       std::vector<double> b((size_t)N_STATE * S);
       download(state_.p + c0 * N_STATE, b.data(), b.size());
       for (int q = 0; q < N_STATE; q++)
-         for (int64_t k = 0; k < n; k++) b[blk_index(q, k, N_STATE)] = st[q * n + k];
+         for (int64_t k = 0; k < n; k++) b[run_pair_index(q, k, N_STATE, S)] = st[q * n + k];
       put(state_.p + c0 * N_STATE, b);
    } else {
       std::vector<float> b32((size_t)kMixed32 * S);

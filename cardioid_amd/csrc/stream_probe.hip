@@ -1,15 +1,17 @@
 // stream_probe.hip -- measured HBM stream rates of the fused FP64 kernel's
 // state pattern, for the roofline's practical ceiling (cdh_stream_rate).
 //
-// The pattern: 19 FP64 fields per cell in 64-cell blocks (blk_index), one
-// wavefront per block, one non-temporal 8-byte access per lane and field, as
-// k_engine_f64 streams its states.  Read only, write only, or read and
-// written in place.
+// The pattern: 19 FP64 fields per cell in runs of kRun cells with the
+// fields in pairs (run_pair_index), one wavefront per 64 cells, one
+// non-temporal 16-byte access per lane and pair (8 bytes for the odd
+// field), as k_engine_f64 streams its states.  Read only, write only, or
+// read and written in place.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 #include "../../include/cardioid_hip.h"
+#include "box_access.hpp"
 #include "hip_util.hpp"
 #include "tt06_model.hpp"
 
@@ -24,11 +26,15 @@ __global__ __launch_bounds__(256) void k_stream(double *__restrict__ s, int64_t 
 {
    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
    if (t >= ntile) return;
-   double *p = s + t * (kFields * kBlk) + (threadIdx.x & 63);
+   const int64_t k0 = t * kBlk, r0 = k0 & ~(kRun - 1);
+   const uint32_t rw = (uint32_t)run_width(k0, ntile * kBlk);
+   const __amdgpu_buffer_rsrc_t R = stream_rsrc(s + r0 * kFields);
+   const uint32_t lc = (uint32_t)(k0 - r0 + (threadIdx.x & 63));
    double v[kFields];
    if (PATTERN != 1) {
 #pragma unroll
-      for (int q = 0; q < kFields; q++) v[q] = __builtin_nontemporal_load(p + kBlk * q);
+      for (int q = 0; q + 1 < kFields; q += 2) stream_load_f64x2(R, 16u * lc, 8u * rw * q, v[q], v[q + 1]);
+      v[kFields - 1] = stream_load_f64(R, 8u * lc, 8u * rw * (kFields - 1));
    } else {
 #pragma unroll
       for (int q = 0; q < kFields; q++) v[q] = (double)(t + q);
@@ -42,8 +48,9 @@ __global__ __launch_bounds__(256) void k_stream(double *__restrict__ s, int64_t 
       return;
    }
 #pragma unroll
-   for (int q = 0; q < kFields; q++)
-      __builtin_nontemporal_store(v[q] + acc * 1e-300, p + kBlk * q);
+   for (int q = 0; q + 1 < kFields; q += 2)
+      stream_store_f64x2(v[q] + acc * 1e-300, v[q + 1] + acc * 1e-300, R, 16u * lc, 8u * rw * q);
+   stream_store_f64(v[kFields - 1] + acc * 1e-300, R, 8u * lc, 8u * rw * (kFields - 1));
 }
 
 }  // namespace

@@ -125,8 +125,7 @@ __host__ __device__ inline int64_t grp_index9(int k, int64_t i)
 // takes 0.922 ms with 4096-cell runs against 0.930 ms with 64-cell blocks
 // (0.928 with 1024 or 2048, 0.932 with 8192, 0.949 with 16384): the width
 // changes how the concurrent read-and-write streams fall on the HBM channels,
-// measured rather than modelled.  The FP64 step keeps 64-cell blocks:
-// 1.446 ms against 1.460 ms with 4096-cell runs and 1.541 ms with 8192.
+// measured rather than modelled.
 constexpr int64_t kRun = 4096;
 __host__ __device__ inline int64_t run_width(int64_t k, int64_t S)
 {
@@ -138,6 +137,22 @@ __host__ __device__ inline int64_t run_index(int64_t q, int64_t k, int nq, int64
 {
    const int64_t r0 = k & ~(kRun - 1);
    return r0 * nq + q * run_width(k, S) + (k - r0);
+}
+
+// The FP64 engine's states: per slab, runs of kRun cells as run_index, with
+// the fields in pairs: fields 2p and 2p + 1 of a cell side by side (16
+// bytes per lane), the pairs one after the other, and with nq odd the last
+// field alone after them.  On one MI355X the FP64 FMA step takes 1.432 ms
+// on the 20.1 M-cell shell this way against 1.457 ms with 64-cell blocks
+// and one field per access (reference rounding 1.558 against 1.563), bit
+// for bit the same; runs with one field per access had measured 1.460
+// against 1.446 ms.  The mixed engine's FP32 fields measured slower in
+// pairs and keep run_index.
+__host__ __device__ inline int64_t run_pair_index(int64_t q, int64_t k, int nq, int64_t S)
+{
+   const int64_t r0 = k & ~(kRun - 1);
+   const int64_t w = run_width(k, S), j = k - r0;
+   return q < (nq & ~1) ? r0 * nq + (q & ~1) * w + 2 * j + (q & 1) : r0 * nq + q * w + j;
 }
 
 // Per-cell-type parameters (a reaction object's CellTypeParms).
