@@ -180,13 +180,14 @@ def valu(stem, ncell, ms_per_step, peak_tflops=78.6, simds=1024, clock_ghz=2.4):
 
 def stream_ceiling(hip, cells, reps=20):
     """The device's HBM rates for the fused FP64 kernel's state pattern (19
-    FP64 fields per cell in 64-cell blocks, one 8-byte access per lane and
-    field) over the same number of cells: read only, write only, and read
-    and written in place (cdh_stream_rate).  The 8 TB/s peak is not
+    FP64 fields per cell in 4096-cell runs, the fields in pairs, one 16-byte
+    access per lane and pair) over the same number of cells: read only,
+    write only, and read and written in place (cdh_stream_rate).  The 8 TB/s peak is not
     reachable for writes; these are the practical ceilings."""
     out = {p + "_GBps": hip.stream_rate(p, cells, reps) for p in ("read", "write", "inplace")}
-    out["pattern"] = ("19 FP64 fields per cell in 64-cell blocks, one non-temporal 8-byte access "
-                      f"per lane and field, {cells} cells, {reps} passes each")
+    out["pattern"] = ("19 FP64 fields per cell in 4096-cell runs with the fields in pairs, one "
+                      "non-temporal 16-byte access per lane and pair (8 bytes for the odd field), "
+                      f"as the FP64 step streams its states; {cells} cells, {reps} passes each")
     return out
 
 

@@ -250,7 +250,8 @@ STREAM_PATTERNS = {"read": 0, "write": 1, "inplace": 2}
 
 def stream_rate(pattern: str, cells: int, reps: int = 20) -> float:
     """GB/s of one HBM stream in the fused FP64 kernel's state pattern (19
-    FP64 fields per cell in 64-cell blocks): "read", "write" or "inplace"
+    FP64 fields per cell in 4096-cell runs, the fields in pairs): "read",
+    "write" or "inplace"
     (cdh_stream_rate)."""
     g = C.c_double()
     check(lib().cdh_stream_rate(STREAM_PATTERNS[pattern], int(cells), int(reps), C.byref(g)),

@@ -456,8 +456,9 @@ int cdh_batch_info(cdh_batch *b, int64_t *device_bytes, int64_t *cells_used, int
 /* ---- measurement ----------------------------------------------------------
  * No counterpart in the reference: the practical ceiling for the roofline of
  * the fused FP64 step.  Times `reps` passes of one HBM stream over `cells`
- * cells of 19 FP64 fields in 64-cell blocks, one non-temporal 8-byte access
- * per lane and field (the fused kernel's state pattern): pattern 0 reads,
+ * cells of 19 FP64 fields in 4096-cell runs with the fields in pairs, one
+ * non-temporal 16-byte access per lane and pair (the fused kernel's state
+ * pattern): pattern 0 reads,
  * 1 writes, 2 reads and writes in place.  *gbps = bytes moved / time.
  * Allocates and frees its own buffer (cells * 152 bytes) and stream. */
 int cdh_stream_rate(int pattern, int64_t cells, int reps, double *gbps);
