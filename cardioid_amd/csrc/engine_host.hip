@@ -98,23 +98,28 @@ KernelFn kernel_for(int mode, bool exact, int waves)
 int min_waves(int mode) { return mode == CDH_MODE_MIXED_FUSED ? 3 : 2; }
 int max_waves(int mode) { return mode == CDH_MODE_MIXED_FUSED ? 4 : 3; }
 
-// the same for the mixed engine's states: a device range of S cells in runs
-// (run_index)
+// the engines' states: a slab's [nq][n] array from its device range of S
+// cells in runs, one field per access (run_index) or in pairs
+// (run_pair_index), and back
+inline int64_t state_index(bool pairs, int q, int64_t k, int nq, int64_t S)
+{
+   return pairs ? run_pair_index(q, k, nq, S) : run_index(q, k, nq, S);
+}
 template <class T>
-std::vector<T> from_runs(const std::vector<T> &b, int nq, int64_t n, int64_t S)
+std::vector<T> from_runs(const std::vector<T> &b, int nq, int64_t n, int64_t S, bool pairs)
 {
    std::vector<T> out((size_t)nq * n);
    for (int q = 0; q < nq; q++)
-      for (int64_t k = 0; k < n; k++) out[q * n + k] = b[run_index(q, k, nq, S)];
+      for (int64_t k = 0; k < n; k++) out[q * n + k] = b[state_index(pairs, q, k, nq, S)];
    return out;
 }
 
 template <class T>
-std::vector<T> to_runs(const std::vector<T> &soa, int nq, int64_t S)
+std::vector<T> to_runs(const std::vector<T> &soa, int nq, int64_t S, bool pairs)
 {
    std::vector<T> b((size_t)nq * S);
    for (int q = 0; q < nq; q++)
-      for (int64_t k = 0; k < S; k++) b[run_index(q, k, nq, S)] = soa[q * S + k];
+      for (int64_t k = 0; k < S; k++) b[state_index(pairs, q, k, nq, S)] = soa[q * S + k];
    return b;
 }
 
@@ -484,10 +489,7 @@ int Engine::add(const SlabHost &h)
       for (int64_t k = 0; k < S; k++) bi[k] = (uint32_t)(b0 + h.bi[k]);
       put(bi_.p + c0, bi);
       if (mode_ != CDH_MODE_MIXED_FUSED) {
-         std::vector<double> b((size_t)N_STATE * S);
-         for (int q = 0; q < N_STATE; q++)
-            for (int64_t k = 0; k < S; k++) b[run_pair_index(q, k, N_STATE, S)] = h.state[q * S + k];
-         put(state_.p + c0 * N_STATE, b);
+         put(state_.p + c0 * N_STATE, to_runs(h.state, N_STATE, S, true));
       } else {
          std::vector<float> s32((size_t)kMixed32 * S);
          std::vector<double> s64((size_t)2 * S);
@@ -499,8 +501,8 @@ int Engine::add(const SlabHost &h)
             s64[k] = h.state[S_CA_I * S + k];
             s64[S + k] = h.state[S_NA_I * S + k];
          }
-         put(st32_.p + c0 * kMixed32, to_runs(s32, kMixed32, S));
-         put(st64_.p + c0 * 2, to_runs(s64, 2, S));
+         put(st32_.p + c0 * kMixed32, to_runs(s32, kMixed32, S, false));
+         put(st64_.p + c0 * 2, to_runs(s64, 2, S, false));
       }
       if (h.device_weights) {
          // the weights are built below, after the slab's descriptor, which
@@ -895,17 +897,14 @@ std::vector<double> Engine::state19(int id)
    if (mode_ != CDH_MODE_MIXED_FUSED) {
       std::vector<double> b((size_t)N_STATE * S);
       download(state_.p + c0 * N_STATE, b.data(), b.size());
-      std::vector<double> out((size_t)N_STATE * n);
-      for (int q = 0; q < N_STATE; q++)
-         for (int64_t k = 0; k < n; k++) out[q * n + k] = b[run_pair_index(q, k, N_STATE, S)];
-      return out;
+      return from_runs(b, N_STATE, n, S, true);
    }
    std::vector<float> b32((size_t)kMixed32 * S);
    std::vector<double> b64((size_t)2 * S);
    download(st32_.p + c0 * kMixed32, b32.data(), b32.size());
    download(st64_.p + c0 * 2, b64.data(), b64.size());
-   const std::vector<float> s32 = from_runs(b32, kMixed32, n, S);
-   const std::vector<double> s64 = from_runs(b64, 2, n, S);
+   const std::vector<float> s32 = from_runs(b32, kMixed32, n, S, false);
+   const std::vector<double> s64 = from_runs(b64, 2, n, S, false);
    const std::vector<double> v = vm(id);
    std::vector<double> st((size_t)N_STATE * n, 0.0);
    for (int64_t k = 0; k < n; k++) {

@@ -146,8 +146,9 @@ __host__ __device__ inline int64_t run_index(int64_t q, int64_t k, int nq, int64
 // on the 20.1 M-cell shell this way against 1.457 ms with 64-cell blocks
 // and one field per access (reference rounding 1.558 against 1.563), bit
 // for bit the same; runs with one field per access had measured 1.460
-// against 1.446 ms.  The mixed engine's FP32 fields measured slower in
-// pairs and keep run_index.
+// against 1.446 ms.  The mixed engine's fields measured slower in pairs
+// (its FP32 fields 0.976 against 0.908 ms, its two FP64 ones alone 0.908
+// against 0.902 ms) and keep run_index.
 __host__ __device__ inline int64_t run_pair_index(int64_t q, int64_t k, int nq, int64_t S)
 {
    const int64_t r0 = k & ~(kRun - 1);
