@@ -67,9 +67,11 @@ __device__ __forceinline__ void cell_f64(const EngineArgs &a, const SlabDev &S, 
        (uint32_t)run_width(k0, (int64_t)__builtin_amdgcn_readfirstlane(S.ntile) * kBlk);
    const __amdgpu_buffer_rsrc_t RS = stream_rsrc(a.state + ((int64_t)first * kBlk + r0) * N_STATE);
    const uint32_t lc = (uint32_t)(k0 - r0 + lane);
+   static_assert(N_STATE % 2 == 1, "the last state field stands alone after the pairs");
    double st[N_STATE];
 #pragma unroll
-   for (int s = 0; s + 1 < N_STATE; s += 2) stream_load_f64x2(RS, 16u * lc, 8u * rw * s, st[s], st[s + 1]);
+   for (int s = 0; s + 1 < N_STATE; s += 2)
+      stream_load_f64x2(RS, 16u * lc, 8u * rw * s, st[s], st[s + 1]);
    st[N_STATE - 1] = stream_load_f64(RS, 8u * lc, 8u * rw * (N_STATE - 1));
    const ParamSet P = a.params[v.param_base + (int)(meta & 0xFFu)];
    // the non-gate update in a basic block of its own (gate_mask bit 12,
